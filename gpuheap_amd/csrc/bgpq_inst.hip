@@ -1,0 +1,45 @@
+// One instantiation unit per (key type, variant), selected at compile time
+// with -DBGPQ_KEY_BITS=32|64 and -DBGPQ_VAR=0..3, so the build compiles the
+// 8 units in parallel.
+#include "bgpq_kernels.hpp"
+
+#if BGPQ_KEY_BITS == 32
+using KeyT = uint32_t;
+#define BGPQ_ROWS_FN(v) rows_u32_v##v
+#else
+using KeyT = uint64_t;
+#define BGPQ_ROWS_FN(v) rows_u64_v##v
+#endif
+#define BGPQ_CAT(a, b) BGPQ_CAT2(a, b)
+#define BGPQ_CAT2(a, b) a##b
+
+namespace bgpq {
+
+#if BGPQ_KEY_BITS == 32
+const LaunchRow* BGPQ_CAT(rows_u32_v, BGPQ_VAR)() {
+#else
+const LaunchRow* BGPQ_CAT(rows_u64_v, BGPQ_VAR)() {
+#endif
+    static const LaunchRow rows[kNumBatchSizes] = {
+        Launch<KeyT, BGPQ_VAR, 64>::row(),  Launch<KeyT, BGPQ_VAR, 128>::row(),
+        Launch<KeyT, BGPQ_VAR, 256>::row(), Launch<KeyT, BGPQ_VAR, 512>::row(),
+        Launch<KeyT, BGPQ_VAR, 1024>::row(), Launch<KeyT, BGPQ_VAR, 2048>::row(),
+    };
+    return rows;
+}
+
+#if BGPQ_VAR == 0
+#if BGPQ_KEY_BITS == 32
+hipError_t fill_u32(void* p, long long n, uint32_t v, hipStream_t s) {
+#else
+hipError_t fill_u64(void* p, long long n, uint64_t v, hipStream_t s) {
+#endif
+    long long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_fill<KeyT>, dim3((unsigned)blocks), dim3(256), 0, s, (KeyT*)p, n, (KeyT)v);
+    return hipGetLastError();
+}
+#endif
+
+}  // namespace bgpq

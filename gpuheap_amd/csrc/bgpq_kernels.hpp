@@ -1,0 +1,168 @@
+// Bulk kernels that drive one shared heap from every workgroup of a grid,
+// and the launch table the C ABI dispatches through.
+//
+// Counterparts: insertKernel / deleteKernel / concurrentKernel of
+// test/heap_test.cu:13-74 and test/partial_heap_test.cu:13-77.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "bgpq_device.hpp"
+
+namespace bgpq {
+
+// Workgroup size per batch size.  Every thread holds B/NT keys while sorting
+// and produces 2B/NT keys per merge.
+constexpr int threads_for(int B) {
+    return B <= 128 ? 64 : (B == 256 ? 128 : (B <= 1024 ? 256 : 512));
+}
+
+template <typename K, int VAR, int B>
+__global__ __launch_bounds__(threads_for(B)) void k_insert(HeapView<K> h, const K* keys, long long n,
+                                                           int chunk) {
+    constexpr int NT = threads_for(B);
+    DevHeap<K, VAR, B, NT> hp{h};
+    const long long calls = (n + chunk - 1) / chunk;
+    for (long long i = blockIdx.x; i < calls; i += gridDim.x) {
+        const long long rem = n - i * chunk;
+        hp.insertion(keys + i * chunk, (int)(rem < chunk ? rem : chunk), 0);
+    }
+}
+
+template <typename K, int VAR, int B>
+__global__ __launch_bounds__(threads_for(B)) void k_delete(HeapView<K> h, K* out, long long calls) {
+    constexpr int NT = threads_for(B);
+    DevHeap<K, VAR, B, NT> hp{h};
+    for (long long i = blockIdx.x; i < calls; i += gridDim.x) {
+        int size = 0;
+        if (hp.deleteRoot(out, size)) hp.deleteUpdate(0);
+    }
+}
+
+template <typename K, int VAR, int B>
+__global__ __launch_bounds__(threads_for(B)) void k_concurrent(HeapView<K> h, const K* keys, long long n,
+                                                               int chunk, K* out, long long calls) {
+    constexpr int NT = threads_for(B);
+    DevHeap<K, VAR, B, NT> hp{h};
+    const int half = gridDim.x / 2;
+    if ((int)blockIdx.x < half) {
+        const long long ins = (n + chunk - 1) / chunk;
+        for (long long i = blockIdx.x; i < ins; i += half) {
+            const long long rem = n - i * chunk;
+            hp.insertion(keys + i * chunk, (int)(rem < chunk ? rem : chunk), 0);
+        }
+    } else {
+        const int nd = gridDim.x - half;
+        for (long long i = blockIdx.x - half; i < calls; i += nd) {
+            int size = 0;
+            if (hp.deleteRoot(out, size)) hp.deleteUpdate(0);
+        }
+    }
+}
+
+// Primitive test kernels: one workgroup per batch.
+template <typename K, int B>
+__global__ __launch_bounds__(threads_for(B)) void k_prim_sort(const K* in, const int* sizes, K* out, K pad) {
+    constexpr int NT = threads_for(B);
+    extern __shared__ __align__(16) int lds[];
+    K* X0 = reinterpret_cast<K*>(lds);
+    wg_sort_batch<K, B, NT>(X0, in + (size_t)blockIdx.x * B, sizes[blockIdx.x], pad);
+    copy_batch<K, B, NT>(out + (size_t)blockIdx.x * B, X0);
+}
+
+template <typename K, int B>
+__global__ __launch_bounds__(threads_for(B)) void k_prim_merge(const K* a, const K* b, K* small, K* large) {
+    constexpr int NT = threads_for(B);
+    extern __shared__ __align__(16) int lds[];
+    K* X0 = reinterpret_cast<K*>(lds);
+    K* X1 = X0 + B;
+    const size_t o = (size_t)blockIdx.x * B;
+    copy_batch<K, B, NT>(X0, a + o);
+    copy_batch<K, B, NT>(X1, b + o);
+    __syncthreads();
+    wg_merge<K, B, NT>(X0, X1, ToPtr<K>{small + o}, ToPtr<K>{large + o});
+}
+
+template <typename K>
+__global__ void k_fill(K* p, long long n, K v) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Type-erased launchers, one row per (key type, variant, batch size).
+struct HeapArgs {
+    void* items;
+    int* status;
+    int* ctr;
+    uint64_t init_limits;
+    int batch_num;
+    int heap_sort;
+};
+
+struct LaunchRow {
+    int batch_size;
+    hipError_t (*sort)(const void* in, const int* sizes, void* out, uint64_t pad, int batches, hipStream_t s);
+    hipError_t (*merge)(const void* a, const void* b, void* small, void* large, int batches, hipStream_t s);
+    hipError_t (*insert)(const HeapArgs&, const void* keys, long long n, int chunk, int grid, hipStream_t s);
+    hipError_t (*del)(const HeapArgs&, void* out, long long calls, int grid, hipStream_t s);
+    hipError_t (*conc)(const HeapArgs&, const void* keys, long long n, int chunk, void* out, long long calls,
+                       int grid, hipStream_t s);
+};
+
+template <typename K>
+inline HeapView<K> view_of(const HeapArgs& a) {
+    return HeapView<K>{(K*)a.items, a.status, a.ctr, (K)a.init_limits, a.batch_num, a.heap_sort};
+}
+
+template <typename K, int VAR, int B>
+struct Launch {
+    static constexpr size_t lds = (size_t)delete_lds_bytes<K, B>();
+    static hipError_t insert(const HeapArgs& a, const void* keys, long long n, int chunk, int grid,
+                             hipStream_t s) {
+        hipLaunchKernelGGL((k_insert<K, VAR, B>), dim3(grid), dim3(threads_for(B)), lds, s, view_of<K>(a),
+                           (const K*)keys, n, chunk);
+        return hipGetLastError();
+    }
+    static hipError_t del(const HeapArgs& a, void* out, long long calls, int grid, hipStream_t s) {
+        hipLaunchKernelGGL((k_delete<K, VAR, B>), dim3(grid), dim3(threads_for(B)), lds, s, view_of<K>(a),
+                           (K*)out, calls);
+        return hipGetLastError();
+    }
+    static hipError_t conc(const HeapArgs& a, const void* keys, long long n, int chunk, void* out,
+                           long long calls, int grid, hipStream_t s) {
+        hipLaunchKernelGGL((k_concurrent<K, VAR, B>), dim3(grid), dim3(threads_for(B)), lds, s, view_of<K>(a),
+                           (const K*)keys, n, chunk, (K*)out, calls);
+        return hipGetLastError();
+    }
+    static hipError_t sort(const void* in, const int* sizes, void* out, uint64_t pad, int batches, hipStream_t s) {
+        hipLaunchKernelGGL((k_prim_sort<K, B>), dim3(batches), dim3(threads_for(B)), B * sizeof(K), s, (const K*)in,
+                           sizes, (K*)out, (K)pad);
+        return hipGetLastError();
+    }
+    static hipError_t merge(const void* a, const void* b, void* small, void* large, int batches, hipStream_t s) {
+        hipLaunchKernelGGL((k_prim_merge<K, B>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
+                           (const K*)a, (const K*)b, (K*)small, (K*)large);
+        return hipGetLastError();
+    }
+    static LaunchRow row() { return LaunchRow{B, &sort, &merge, &insert, &del, &conc}; }
+};
+
+constexpr int kNumBatchSizes = 6;  // 64 .. 2048
+inline int batch_index(int B) {
+    switch (B) {
+        case 64: return 0;
+        case 128: return 1;
+        case 256: return 2;
+        case 512: return 3;
+        case 1024: return 4;
+        case 2048: return 5;
+        default: return -1;
+    }
+}
+
+// Defined in the per-(key, variant) instantiation units.
+const LaunchRow* rows_u32(int var);
+const LaunchRow* rows_u64(int var);
+hipError_t fill_u32(void* p, long long n, uint32_t v, hipStream_t s);
+hipError_t fill_u64(void* p, long long n, uint64_t v, hipStream_t s);
+
+}  // namespace bgpq

@@ -1,0 +1,186 @@
+"""GPU parity of the heap against the CPU oracle, through the C ABI.
+
+* One workgroup (grid=1) performs the oracle model's data movement exactly:
+  the whole device state (every slot, every counter) must equal the model's
+  after every operation, for all four variants and both key types.
+* Many workgroups: the reference's own harness (test/heap_test.cu,
+  test/partial_heap_test.cu) -- insert half, concurrent insert+delete, delete
+  the rest -- must emit numpy's sort of the keys, bit for bit, with
+  checkInsertHeap holding after the insertion phase.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import gpuheap_amd as G
+from gpuheap_amd import DeviceBuffer, Heap, batch_num_for
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = list(G.VARIANTS)
+PB = ("buitdd", "tditdd")
+INIT = {"u32": O.U32_MAX, "u64": O.U64_MAX}
+
+
+def _state_equal(h, m, key):
+    items, status, ctr = h.snapshot()
+    bc, pbs, dc, err = m.counters()
+    assert ctr[3] == 0 and err == 0, f"error flags gpu={ctr[3]} model={err}"
+    assert (ctr[0], ctr[1], ctr[2]) == (bc, pbs, dc)
+    assert not status.any(), "a status word is not AVAIL after the operation"
+    want = m.items()
+    got = items.astype(np.uint64)
+    if not np.array_equal(got, want):
+        bad = np.nonzero(got != want)[0]
+        raise AssertionError(f"slot mismatch at keys {bad[:8]} (slot {bad[0] // m.B})")
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("B", [64, 128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("key", ["u32", "u64"])
+def test_single_workgroup_state_parity(gpu, variant, B, key):
+    print(f"[case] state parity {variant} B={B} {key}", flush=True)
+    rng = np.random.default_rng(B * 7 + VARIANTS.index(variant))
+    batch_num = 128
+    h = Heap(batch_num, B, variant=variant, key=key)
+    m = O.Model(variant, batch_num, B, INIT[key])
+    dtype = np.uint32 if key == "u32" else np.uint64
+    cap = 4096 * B
+    d_out = DeviceBuffer(cap * np.dtype(dtype).itemsize)
+    held = 0
+    for step in range(80):
+        if held == 0 or (held < 40 * B and rng.random() < 0.6):
+            size = int(rng.integers(1, B + 1)) if variant in PB else B
+            hi = 2**32 - 1 if key == "u32" else 2**64 - 1
+            keys = rng.integers(0, hi, size=size, dtype=np.uint64)
+            if step % 5 == 0:
+                keys %= np.uint64(9)  # duplicates
+            keys = keys.astype(dtype)
+            d = DeviceBuffer.from_numpy(keys)
+            h.insert(d, size, chunk=size, grid=1)
+            assert m.insert(keys.astype(np.uint64)) == 0
+            held += size
+        else:
+            h.delete(d_out, 1, grid=1)
+            held -= m.delete()
+        _state_equal(h, m, key)
+    dc = m.counters()[2]
+    got = d_out.to_numpy(dtype, dc).astype(np.uint64)
+    assert np.array_equal(got, m.out[:dc])
+
+
+def _harness(variant, B, n, key, seed, grid):
+    """test/heap_test.cu main (:85-172) and test/partial_heap_test.cu: returns
+    (heap output, expected, check-after-insert)."""
+    if key == "u32":
+        keys = O.harness_keys(seed, n)
+    else:
+        k32 = O.harness_keys(seed, n).astype(np.uint64)
+        keys = (k32 << np.uint64(32)) | np.arange(n, dtype=np.uint64)
+    half = n // 2
+    bn = batch_num_for(n, B)
+    h = Heap(bn, B, variant=variant, key=key)
+    pb = variant in PB
+    chunk1 = max(1, B // 3) if pb else B
+    chunk2 = max(1, 2 * B // 3) if pb else B
+    d_in = DeviceBuffer.from_numpy(keys)
+    d_out = DeviceBuffer(keys.nbytes)
+    h.insert(d_in, half, chunk=chunk1, grid=grid)
+    ok = h.checkInsertHeap()
+    msg = getattr(h, "last_check_message", "")
+    calls = (half + B - 1) // B
+    second = DeviceBuffer.from_numpy(keys[half:])
+    h.concurrent(second, n - half, d_out, calls, chunk=chunk2, grid=grid)
+    h.delete(d_out, calls + 1, grid=grid)
+    G.sync()
+    assert h.error_flags() == 0, f"device error flags {h.error_flags()}"
+    assert h.isEmpty()
+    return d_out.to_numpy(keys.dtype, n), np.sort(keys), (ok, msg)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("B", [128, 256, 512, 1024])
+def test_reference_harness_matrix(gpu, variant, B):
+    # test/test.sh:5-34: batch 128..1024, 2^20 keys (2^21, 2^22 in the slow test)
+    print(f"[case] harness {variant} B={B}", flush=True)
+    got, want, (ok, msg) = _harness(variant, B, 1 << 20, "u32", seed=B, grid=32)
+    assert ok, msg
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_reference_harness_u64(gpu, variant):
+    print(f"[case] harness u64 {variant}", flush=True)
+    got, want, (ok, msg) = _harness(variant, 512, 1 << 19, "u64", seed=3, grid=64)
+    assert ok, msg
+    assert np.array_equal(got, want)
+
+
+def test_config1_buitdd_heapsort(gpu):
+    """2^20 uniform u32 keys (SplitMix64 seed 0), buitdd, B=1024,
+    batch_num 2048: insert all in B-key calls, checkInsertHeap, delete until
+    empty; output equals the sorted keys."""
+    print("[case] config1", flush=True)
+    n, B = 1 << 20, 1024
+    keys = O.keys_u32(0, n)
+    h = Heap(batch_num_for(n, B), B, variant="buitdd")
+    assert h.batch_num == 2048
+    d = DeviceBuffer.from_numpy(keys)
+    h.insert(d, n, chunk=B)
+    assert h.checkInsertHeap(), h.last_check_message
+    assert h.itemCount() == n and h.nodeCount() == n // B
+    out = DeviceBuffer(keys.nbytes)
+    h.delete(out, n // B + 1)
+    G.sync()
+    assert h.isEmpty() and h.error_flags() == 0
+    assert np.array_equal(out.to_numpy(np.uint32, n), np.sort(keys))
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("order", ["random", "descending"])
+def test_heapsort_api(gpu, variant, order):
+    print(f"[case] heapsort {variant} {order}", flush=True)
+    n, B = 1 << 18, 512
+    keys = O.keys_u32(9, n)
+    if order == "descending":
+        keys = np.sort(keys)[::-1].copy()
+    h = Heap(batch_num_for(n, B), B, variant=variant)
+    d = DeviceBuffer.from_numpy(keys)
+    t_ins, t_del = h.heapsort(d, n)
+    assert t_ins > 0 and t_del > 0
+    assert np.array_equal(d.to_numpy(np.uint32, n), np.sort(keys))
+
+
+@pytest.mark.parametrize("variant", PB)
+def test_partial_only_and_small_inputs(gpu, variant):
+    # fewer keys than a batch: they live in the partial buffer only, and one
+    # deleteRoot returns all of them (buitdd:305-320)
+    B = 256
+    for n in (1, 5, B - 1, B + 1, 3 * B + 17):
+        keys = O.keys_u32(n, n)
+        h = Heap(64, B, variant=variant)
+        d = DeviceBuffer.from_numpy(keys)
+        h.insert(d, n, chunk=B)
+        assert h.checkInsertHeap(), h.last_check_message
+        assert h.itemCount() == n
+        out = DeviceBuffer(keys.nbytes)
+        h.delete(out, n // B + 2, grid=4)
+        G.sync()
+        assert h.isEmpty()
+        assert np.array_equal(out.to_numpy(np.uint32, n), np.sort(keys))
+
+
+def test_empty_delete_and_errors(gpu):
+    h = Heap(16, 256, variant="buitdd_np")
+    out = DeviceBuffer(256 * 4)
+    h.delete(out, 3)  # deleting from an empty heap returns nothing
+    G.sync()
+    assert h.isEmpty() and h.error_flags() == 0
+    keys = DeviceBuffer.from_numpy(np.arange(100, dtype=np.uint32))
+    with pytest.raises(G.BgpqError):
+        h.insert(keys, 100, chunk=100)  # _np variants take full batches only
+    with pytest.raises(G.BgpqError):
+        Heap(16, 1000)  # batch size must be a power of two
+    big = Heap(4, 256, variant="buitdd")
+    with pytest.raises(G.BgpqError):
+        big.heapsort(DeviceBuffer(4 * 256 * 4), 4 * 256)  # needs batch_num >= 2 * batches

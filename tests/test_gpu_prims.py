@@ -1,0 +1,66 @@
+"""The batch primitives on the GPU against numpy, for every built batch size:
+the insert path's pad-and-sort (ibitonicSort, heap/heaputil.cuh:23-47) must
+equal numpy's sort of the padded batch, and the merge (imergePath,
+heap/heaputil.cuh:99-158) must split the sorted union at B."""
+import numpy as np
+import pytest
+
+import gpuheap_amd as G
+from gpuheap_amd import DeviceBuffer
+from gpuheap_amd._lib import check, lib
+
+pytestmark = pytest.mark.gpu
+
+BATCHES = [64, 128, 256, 512, 1024, 2048]
+KEYS = {"u32": (0, np.uint32), "u64": (1, np.uint64)}
+
+
+def _rand(rng, dtype, shape, dup_every=3):
+    info = np.iinfo(dtype)
+    k = rng.integers(0, int(info.max), size=shape, dtype=np.uint64).astype(dtype)
+    k[::dup_every] %= dtype(13)  # many duplicates
+    return k
+
+
+@pytest.mark.parametrize("B", BATCHES)
+@pytest.mark.parametrize("key", ["u32", "u64"])
+def test_sort_batch(gpu, B, key):
+    kt, dtype = KEYS[key]
+    rng = np.random.default_rng(B)
+    nb = 64
+    keys = _rand(rng, dtype, (nb, B))
+    sizes = rng.integers(1, B + 1, size=nb).astype(np.int32)
+    sizes[:4] = [B, 1, B - 1, B // 2]
+    pad = int(np.iinfo(dtype).max)
+    d_in, d_sz = DeviceBuffer.from_numpy(keys), DeviceBuffer.from_numpy(sizes)
+    d_out = DeviceBuffer(keys.nbytes)
+    check(lib().bgpq_test_sort(kt, B, d_in.ptr, d_sz.ptr, d_out.ptr, pad, nb))
+    got = d_out.to_numpy(dtype).reshape(nb, B)
+    for i in range(nb):
+        want = np.full(B, pad, dtype=dtype)
+        want[:sizes[i]] = keys[i, :sizes[i]]
+        assert np.array_equal(got[i], np.sort(want)), f"batch {i} size {sizes[i]}"
+
+
+@pytest.mark.parametrize("B", BATCHES)
+@pytest.mark.parametrize("key", ["u32", "u64"])
+def test_merge_batches(gpu, B, key):
+    kt, dtype = KEYS[key]
+    rng = np.random.default_rng(B + 1)
+    nb = 64
+    a = np.sort(_rand(rng, dtype, (nb, B)), axis=1)
+    b = np.sort(_rand(rng, dtype, (nb, B), dup_every=5), axis=1)
+    # disjoint, interleaved and identical batches
+    b[0] = a[0]
+    b[1] = a[1] + dtype(1)
+    a[2], b[2] = np.sort(np.arange(B, dtype=dtype)), np.sort(np.arange(B, 2 * B, dtype=dtype))
+    a[3], b[3] = b[2].copy(), a[2].copy()
+    d = [DeviceBuffer.from_numpy(x) for x in (a, b)]
+    small, large = DeviceBuffer(a.nbytes), DeviceBuffer(a.nbytes)
+    check(lib().bgpq_test_merge(kt, B, d[0].ptr, d[1].ptr, small.ptr, large.ptr, nb))
+    s = small.to_numpy(dtype).reshape(nb, B)
+    l_ = large.to_numpy(dtype).reshape(nb, B)
+    for i in range(nb):
+        u = np.sort(np.concatenate([a[i], b[i]]))
+        assert np.array_equal(s[i], u[:B]), f"small half of batch {i}"
+        assert np.array_equal(l_[i], u[B:]), f"large half of batch {i}"
