@@ -40,7 +40,7 @@ SIGNATURES = {
     "bgpq_lds_bytes": (c_int, [c_int, c_int]),
     "bgpq_default_grid": (c_int, [c_int]),
     "bgpq_test_sort": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_u64, c_int]),
-    "bgpq_test_merge": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "bgpq_test_merge": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int]),
     "bgpq_last_error": (ctypes.c_char_p, []),
     "bgpq_dev_malloc": (c_int, [ctypes.POINTER(c_void_p), c_size_t]),
     "bgpq_dev_free": (c_int, [c_void_p]),

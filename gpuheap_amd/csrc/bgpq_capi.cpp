@@ -76,6 +76,10 @@ static const LaunchRow* rows_for(int key_type, int variant) {
     return nullptr;
 }
 
+// Status words and counters sit kWordStride ints apart (one 128-byte line each).
+static size_t status_bytes(int batch_num) { return sizeof(int) * kWordStride * (size_t)(batch_num + 1); }
+static size_t ctr_bytes() { return sizeof(int) * kWordStride * CTR_WORDS; }
+
 static HeapArgs args_of(const bgpq_heap* h) {
     return HeapArgs{h->items, h->status, h->ctr, h->init_limits, h->batch_num, 1};
 }
@@ -85,13 +89,23 @@ static int fill_items(bgpq_heap* h, hipStream_t s) {
     hipError_t e = h->key_type == BGPQ_KEY_U64 ? fill_u64(h->items, n, h->init_limits, s)
                                                : fill_u32(h->items, n, (uint32_t)h->init_limits, s);
     if (e != hipSuccess) return fail(BGPQ_EHIP, "fill: %s", hipGetErrorString(e));
-    HIPCHK(hipMemsetAsync(h->status, 0, sizeof(int) * (size_t)(h->batch_num + 1), s));
-    HIPCHK(hipMemsetAsync(h->ctr, 0, sizeof(int) * CTR_WORDS, s));
+    HIPCHK(hipMemsetAsync(h->status, 0, status_bytes(h->batch_num), s));
+    HIPCHK(hipMemsetAsync(h->ctr, 0, ctr_bytes(), s));
     return BGPQ_OK;
 }
 
 static int counters(bgpq_heap* h, int* c) {
-    HIPCHK(hipMemcpy(c, h->ctr, sizeof(int) * CTR_WORDS, hipMemcpyDeviceToHost));
+    std::vector<int> raw((size_t)kWordStride * CTR_WORDS);
+    HIPCHK(hipMemcpy(raw.data(), h->ctr, ctr_bytes(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < CTR_WORDS; ++i) c[i] = raw[(size_t)i * kWordStride];
+    return BGPQ_OK;
+}
+
+// status words 0..count-1 gathered from the padded array
+static int status_words(bgpq_heap* h, int* out, int count) {
+    std::vector<int> raw((size_t)kWordStride * count);
+    HIPCHK(hipMemcpy(raw.data(), h->status, sizeof(int) * raw.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < count; ++i) out[i] = raw[(size_t)i * kWordStride];
     return BGPQ_OK;
 }
 
@@ -112,7 +126,8 @@ static int check_impl(bgpq_heap* h) {
     std::vector<K> items((size_t)(nb + 1) * B);
     std::vector<int> status(nb + 1);
     HIPCHK(hipMemcpy(items.data(), h->items, sizeof(K) * items.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(status.data(), h->status, sizeof(int) * status.size(), hipMemcpyDeviceToHost));
+    rc = status_words(h, status.data(), nb + 1);
+    if (rc) return rc;
     const bool bu = h->variant == BGPQ_BUITDD || h->variant == BGPQ_BUITDD_NP;
     if (status[0] != 0) return fail(0, "partial batch state %d, expected AVAIL", status[0]);
     if (nb != 0 && pbs != 0) {
@@ -183,8 +198,8 @@ int bgpq_create(bgpq_heap** out, int variant, int key_type, int batch_num, int b
         return fine ? hipExtMallocWithFlags(p, n, hipDeviceMallocFinegrained) : hipMalloc(p, n);
     };
     hipError_t e = alloc(&h->items, bytes);
-    if (e == hipSuccess) e = alloc((void**)&h->status, sizeof(int) * (size_t)(batch_num + 1));
-    if (e == hipSuccess) e = alloc((void**)&h->ctr, sizeof(int) * CTR_WORDS);
+    if (e == hipSuccess) e = alloc((void**)&h->status, status_bytes(batch_num));
+    if (e == hipSuccess) e = alloc((void**)&h->ctr, ctr_bytes());
     if (e != hipSuccess) {
         bgpq_destroy(h);
         return fail(BGPQ_EHIP, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
@@ -215,7 +230,8 @@ int bgpq_reset(bgpq_heap* h, void* stream) {
 int bgpq_error_flags(bgpq_heap* h, int* out) {
     if (!h || !out) return fail(BGPQ_EINVAL, "null argument");
     int c[CTR_WORDS];
-    HIPCHK(hipMemcpy(c, h->ctr, sizeof c, hipMemcpyDeviceToHost));
+    const int rc = counters(h, c);
+    if (rc) return rc;
     *out = c[CTR_ERROR];
     return BGPQ_OK;
 }
@@ -254,9 +270,14 @@ int bgpq_snapshot(bgpq_heap* h, void* h_items, int* h_status, int* h_counters) {
         HIPCHK(hipMemcpy(h_items, h->items,
                          key_bytes(h->key_type) * (size_t)h->batch_size * (size_t)(h->batch_num + 1),
                          hipMemcpyDeviceToHost));
-    if (h_status)
-        HIPCHK(hipMemcpy(h_status, h->status, sizeof(int) * (size_t)(h->batch_num + 1), hipMemcpyDeviceToHost));
-    if (h_counters) HIPCHK(hipMemcpy(h_counters, h->ctr, sizeof(int) * CTR_WORDS, hipMemcpyDeviceToHost));
+    if (h_status) {
+        const int rc = status_words(h, h_status, h->batch_num + 1);
+        if (rc) return rc;
+    }
+    if (h_counters) {
+        const int rc = counters(h, h_counters);
+        if (rc) return rc;
+    }
     return BGPQ_OK;
 }
 
@@ -331,7 +352,7 @@ int bgpq_heapsort(bgpq_heap* h, void* d_keys, int64_t n, int chunk, int grid_ins
     if ((n + B - 1) / B > (int64_t)h->batch_num / 2)
         return fail(BGPQ_EINVAL, "capacity: %lld keys need batch_num >= %lld", (long long)n,
                     (long long)(2 * ((n + B - 1) / B)));
-    HIPCHK(hipMemsetAsync(h->ctr + CTR_DELETE_COUNT, 0, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(h->ctr + CTR_DELETE_COUNT * kWordStride, 0, sizeof(int), s));
     hipEvent_t e0, e1, e2;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -371,11 +392,11 @@ int bgpq_test_sort(int key_type, int batch_size, const void* d_in, const int* d_
 }
 
 int bgpq_test_merge(int key_type, int batch_size, const void* d_a, const void* d_b, void* d_small, void* d_large,
-                    int batches) {
+                    int batches, int algo) {
     const int bi = batch_index(batch_size);
     const LaunchRow* rows = rows_for(key_type, 0);
     if (bi < 0 || !rows || batches < 1) return fail(BGPQ_EINVAL, "bad primitive test arguments");
-    hipError_t e = rows[bi].merge(d_a, d_b, d_small, d_large, batches, nullptr);
+    hipError_t e = rows[bi].merge(d_a, d_b, d_small, d_large, batches, algo, nullptr);
     if (e != hipSuccess) return fail(BGPQ_EHIP, "merge launch: %s", hipGetErrorString(e));
     HIPCHK(hipDeviceSynchronize());
     return BGPQ_OK;

@@ -9,13 +9,15 @@
 // Every thread of the workgroup must call them.  `smemOffset` counts ints
 // into the kernel's dynamic LDS (`extern __shared__ int[]`), exactly as in
 // the reference; it must be a multiple of 4.  LDS needed past the offset:
-// insertion 2*B*sizeof(K), deleteUpdate 3*B*sizeof(K) (lds_bytes()).
+// insertion 2*B*sizeof(K), deleteUpdate 3*B*sizeof(K).
 //
 // Storage (HBM): items[(batch_num+1)*B]: slot 0 is the partial buffer,
 // slots 1..batchCount the full sorted batches, every other slot holds
-// init_limits.  status[batch_num+1] holds one state word per slot.
-// counters: [0] batchCount, [1] partialBufferSize, [2] deleteCount (output
-// cursor in heap-sort mode), [3] error flags.
+// init_limits.  One status word per slot and the counters (batchCount,
+// partialBufferSize, deleteCount = heap-sort output cursor, error flags,
+// abort word, insertion ticket) each sit alone in a 128-byte line
+// (kWordStride ints apart), so spinning workgroups do not share a line
+// with the words the lock holder updates.
 //
 // Variants (template parameter VAR):
 //   VAR_BUITDD    bottom-up insertion, top-down deletion, partial buffer
@@ -34,8 +36,11 @@ enum { CTR_BATCH_COUNT = 0, CTR_PARTIAL = 1, CTR_DELETE_COUNT = 2, CTR_ERROR = 3
        CTR_WORDS = 8 };
 enum { ERR_OVERFLOW = 1, ERR_BAD_SIZE = 2, ERR_LOST_KEYS = 4, ERR_SPIN_LIMIT = 8 };
 
-// A lock wait longer than this many polls (each at least one HBM atomic
-// round trip, i.e. seconds in total) trips the heap's abort word: every
+// Ints between consecutive status words / counters: one 128-byte line each.
+constexpr int kWordStride = 32;
+
+// A lock wait longer than this many polls (each at least one device-scope
+// atomic, i.e. seconds in total) trips the heap's abort word: every
 // workgroup then leaves its current operation at the next hand-off point, so
 // a protocol fault ends the kernel with ERR_SPIN_LIMIT instead of hanging.
 constexpr int kSpinLimit = 1 << 22;
@@ -43,8 +48,8 @@ constexpr int kSpinLimit = 1 << 22;
 template <typename K>
 struct HeapView {
     K* items;
-    int* status;
-    int* ctr;
+    int* status;  // (batch_num + 1) * kWordStride ints
+    int* ctr;     // CTR_WORDS * kWordStride ints
     K init_limits;
     int batch_num;
     int heap_sort;  // nonzero: deleteRoot writes at items + deleteCount (HEAP_SORT mode)
@@ -70,6 +75,8 @@ __device__ __forceinline__ int next_toward(int cur, int target) {
     return target >> (__builtin_clz((uint32_t)cur) - __builtin_clz((uint32_t)target) - 1);
 }
 
+// LDS words through which lane 0 broadcasts protocol decisions.
+enum { BC_A = 0, BC_B = 1, BC_C = 2, BC_D = 3, BC_E = 4, BC_F = 5, BC_TAG = 6, BC_ABORT = 7 };
 __device__ __forceinline__ int* bcast_slots() {
     __shared__ int slots[8];
     return slots;
@@ -101,16 +108,16 @@ struct DevHeap {
     HeapView<K> h;
 
     __device__ __forceinline__ K* node(int i) const { return h.items + (size_t)i * B; }
-    __device__ __forceinline__ int* st(int i) const { return h.status + i; }
-    __device__ __forceinline__ int* ctr(int i) const { return h.ctr + i; }
+    __device__ __forceinline__ int* st(int i) const { return h.status + (size_t)i * kWordStride; }
+    __device__ __forceinline__ int* ctr(int i) const { return h.ctr + i * kWordStride; }
     __device__ __forceinline__ K* lds_ws(int smemOffset) const {
         extern __shared__ __align__(16) int bgpq_dyn_lds[];
         return reinterpret_cast<K*>(bgpq_dyn_lds + smemOffset);
     }
 
-    // ---- lane-0 lock primitives (only thread 0 calls these).  Each returns
-    // false when the heap's abort word is set or its own wait exceeds
-    // kSpinLimit; the caller then leaves the operation (see aborted()).
+    // ---- lane-level lock primitives (called by one lane each).  Every spin
+    // reads before it writes (test-and-test-and-set) and returns false / -1
+    // when the heap's abort word is set or its own wait exceeds kSpinLimit.
     __device__ __forceinline__ bool poll_abort(int& n) const {
         if ((++n & 63) != 0) return false;
         if (n > kSpinLimit || a_load(ctr(CTR_ABORT)) != 0) {
@@ -122,11 +129,11 @@ struct DevHeap {
     __device__ void trip() const {
         a_store(ctr(CTR_ABORT), 1);
         __hip_atomic_fetch_or(ctr(CTR_ERROR), (int)ERR_SPIN_LIMIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        bcast_slots()[7] = 1;
+        bcast_slots()[BC_ABORT] = 1;
     }
     __device__ __forceinline__ bool lock_cas(int i, int from, int to) const {
         int n = 0;
-        while (!a_cas(st(i), from, to)) {
+        while (!(a_load(st(i)) == from && a_cas(st(i), from, to))) {
             if (poll_abort(n)) return false;
             backoff();
         }
@@ -135,40 +142,49 @@ struct DevHeap {
     // bu "lock whatever state, remember it" idiom (buitdd:372-377,409-418);
     // returns the previous state, or -1 on abort
     __device__ __forceinline__ int lock_max(int i) const {
-        int p, n = 0;
-        while ((p = a_max(st(i), INUSE)) == INUSE) {
+        int n = 0;
+        while (true) {
+            if (a_load(st(i)) != INUSE) {
+                const int p = a_max(st(i), INUSE);
+                if (p != INUSE) return p;
+            }
             if (poll_abort(n)) return -1;
             backoff();
         }
-        return p;
     }
-    // td child lock: CAS AVAIL->INUSE, waiting while INUSE; returns the state
+    // td child lock: AVAIL -> INUSE, waiting while INUSE; returns the state
     // found (AVAIL means locked), or -1 on abort
     __device__ __forceinline__ int lock_child_td(int i) const {
-        int s, n = 0;
-        while ((s = a_cas_val(st(i), AVAIL, INUSE)) == INUSE) {
+        int n = 0;
+        while (true) {
+            const int s = a_load(st(i));
+            if (s != INUSE) {
+                if (s != AVAIL) return s;
+                if (a_cas(st(i), AVAIL, INUSE)) return AVAIL;
+            }
             if (poll_abort(n)) return -1;
             backoff();
         }
-        return s;
     }
     // Uniform abort test after a hand-off barrier.
-    __device__ __forceinline__ bool aborted() const { return bcast_slots()[7] != 0; }
+    __device__ __forceinline__ bool aborted() const { return bcast_slots()[BC_ABORT] != 0; }
     // Start of every operation: thread 0 samples the abort word, all threads
     // agree on it through LDS.
     __device__ __forceinline__ bool op_begin() const {
-        if (threadIdx.x == 0) bcast_slots()[7] = a_load(ctr(CTR_ABORT)) != 0;
+        if (threadIdx.x == 0) bcast_slots()[BC_ABORT] = a_load(ctr(CTR_ABORT)) != 0;
         __syncthreads();
         return !aborted();
     }
-
-    // ---- workgroup hand-off points
-    __device__ __forceinline__ void sync_acq() const {
-        __syncthreads();
-        fence_acq();
+    // Status-word release by one lane: its own earlier stores (counters
+    // written under status[0]) are acknowledged first.
+    __device__ __forceinline__ void release_st(int i, int v) const {
+        wait_stores();
+        a_store(st(i), v);
     }
-    __device__ __forceinline__ void sync_rel() const {
-        fence_rel();
+    // Every wave's stores to heap memory are acknowledged, then a barrier:
+    // after this, lane 0 may publish with a status store.
+    __device__ __forceinline__ void publish() const {
+        wait_stores();
         __syncthreads();
     }
     __device__ __forceinline__ void flag_error(int e) const {
@@ -188,15 +204,6 @@ struct DevHeap {
         else return cur >> 1;
     }
 
-    // Copy `n` keys (any alignment) between global buffers, optionally
-    // resetting the source to init_limits.
-    __device__ __forceinline__ void copy_keys(K* dst, K* src, int n, bool reset) const {
-        for (int i = threadIdx.x; i < n; i += NT) {
-            dst[i] = src[i];
-            if (reset) src[i] = h.init_limits;
-        }
-    }
-
     // ------------------------------------------------------------- deleteRoot
     // Lock the partial buffer (status[0]); if the heap holds full batches,
     // also lock the root and hand the root batch to the caller.  Returns true
@@ -208,7 +215,6 @@ struct DevHeap {
         if (threadIdx.x == 0) {
             do {
                 if (!lock_cas(0, AVAIL, INUSE)) break;
-                fence_acq();
                 const int nb = a_load(ctr(CTR_BATCH_COUNT));
                 const int pbs = a_load(ctr(CTR_PARTIAL));
                 const int off = h.heap_sort ? a_load(ctr(CTR_DELETE_COUNT)) : 0;
@@ -220,34 +226,41 @@ struct DevHeap {
                     if (!lock_cas(1, AVAIL, INUSE)) break;
                     if (h.heap_sort) a_store(ctr(CTR_DELETE_COUNT), off + B);
                 }
-                bc[0] = mode;
-                bc[1] = pbs;
-                bc[2] = off;
+                bc[BC_A] = mode;
+                bc[BC_B] = pbs;
+                bc[BC_C] = off;
             } while (0);
         }
-        sync_acq();
+        __syncthreads();
         if (aborted()) return false;
-        const int mode = bc[0], pbs = bc[1], off = bc[2];
+        const int mode = bc[BC_A], pbs = bc[BC_B], off = bc[BC_C];
         __syncthreads();
         if (mode == 0) {
-            if (threadIdx.x == 0) a_store_rel(st(0), AVAIL);
+            if (threadIdx.x == 0) release_st(0, AVAIL);
             return false;
         }
         if (mode == 1) {
             // only the partial buffer holds keys: hand all of it out (buitdd:305-320)
-            copy_keys(items + off, node(0), pbs, true);
-            sync_rel();
+            K* pb = node(0);
+            for (int i = threadIdx.x; i < pbs; i += NT) {
+                items[off + i] = gload_key(pb + i);
+                gstore_key(pb + i, h.init_limits);
+            }
+            publish();
             if (threadIdx.x == 0) {
                 if (h.heap_sort) a_store(ctr(CTR_DELETE_COUNT), off + pbs);
                 a_store(ctr(CTR_PARTIAL), 0);
-                a_store_rel(st(0), AVAIL);
+                release_st(0, AVAIL);
             }
             size = pbs;
             return false;
         }
         K* dst = items + off;
-        if (((uintptr_t)dst % 16) == 0) copy_batch<K, B, NT>(dst, node(1));
-        else copy_keys(dst, node(1), B, false);
+        if (((uintptr_t)dst % 16) == 0) {
+            g2g_batch<K, B, NT>(dst, node(1));
+        } else {
+            for (int i = threadIdx.x; i < B; i += NT) dst[i] = gload_key(node(1) + i);
+        }
         size = B;
         __syncthreads();
         return true;
@@ -264,24 +277,24 @@ struct DevHeap {
         if (threadIdx.x == 0) {
             const int last = a_add(ctr(CTR_BATCH_COUNT), -1);
             if (last == 1) {
-                a_store_rel(st(1), AVAIL);
-                a_store_rel(st(0), AVAIL);
+                release_st(1, AVAIL);
+                release_st(0, AVAIL);
             }
-            bc[0] = last;
+            bc[BC_A] = last;
         }
         __syncthreads();
-        const int last = bc[0];
+        const int last = bc[BC_A];
         __syncthreads();
         if (last == 1) return;
 
         // ---- take the last batch as the candidate
         if constexpr (BU) {
             if (threadIdx.x == 0) lock_max(last);
-            sync_acq();
+            __syncthreads();
             if (aborted()) return;
             take_batch<K, B, NT>(X0, node(last), h.init_limits);
-            sync_rel();
-            if (threadIdx.x == 0) a_store_rel(st(last), AVAIL);
+            publish();
+            if (threadIdx.x == 0) release_st(last, AVAIL);
         } else {
             // td: the last slot may be an insert target still in flight; then
             // mark it and let the inserter hand its batch over through the
@@ -289,38 +302,37 @@ struct DevHeap {
             if (threadIdx.x == 0) {
                 int type = -1, n = 0;
                 while (true) {
-                    if (a_cas(st(last), AVAIL, INUSE)) { type = 0; break; }
-                    if (a_cas(st(last), TARGET, MARKED)) { type = 1; break; }
+                    const int s = a_load(st(last));
+                    if (s == AVAIL && a_cas(st(last), AVAIL, INUSE)) { type = 0; break; }
+                    if (s == TARGET && a_cas(st(last), TARGET, MARKED)) { type = 1; break; }
                     if (poll_abort(n)) break;
                     backoff();
                 }
-                if (type == 1) {
-                    if (!lock_cas(last, TARGET, AVAIL)) type = -1;
-                }
-                bc[1] = type;
+                if (type == 1 && !lock_cas(last, TARGET, AVAIL)) type = -1;
+                bc[BC_B] = type;
             }
-            sync_acq();
+            __syncthreads();
             if (aborted()) return;
-            const int type = bc[1];
+            const int type = bc[BC_B];
             if (type == 1) {
-                copy_batch<K, B, NT>(X0, node(1));
-                __syncthreads();
+                g2l_batch<K, B, NT>(X0, node(1));
             } else {
                 take_batch<K, B, NT>(X0, node(last), h.init_limits);
-                sync_rel();
-                if (threadIdx.x == 0) a_store_rel(st(last), AVAIL);
+                publish();
+                if (threadIdx.x == 0) release_st(last, AVAIL);
             }
         }
         __syncthreads();
 
         // ---- fold the partial buffer in: the larger half stays there
         if constexpr (PB) {
-            copy_batch<K, B, NT>(X1, node(0));
+            g2l_batch<K, B, NT>(X1, node(0));
             __syncthreads();
-            wg_merge<K, B, NT>(X0, X1, ToPtr<K>{X0}, ToPtr<K>{node(0)});
+            wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
+            wait_stores();
+            __syncthreads();
         }
-        sync_rel();
-        if (threadIdx.x == 0) a_store_rel(st(0), AVAIL);
+        if (threadIdx.x == 0) release_st(0, AVAIL);
 
         // ---- sift the candidate down from the root
         int cur = 1;
@@ -328,14 +340,19 @@ struct DevHeap {
         while (true) {
             const int l = left_child(cur), r = right_child(cur);
             const bool lin = l <= h.batch_num, rin = r <= h.batch_num;
-            if (threadIdx.x == 0) {
-                int lp = AVAIL, rp = AVAIL, type = 2;
-                if constexpr (BU) {
-                    if (lin) lp = lock_max(l);
-                    if (lp >= 0 && rin) rp = lock_max(r);
-                    if (is_inshold(lp)) lp = DELMOD;
-                    if (is_inshold(rp)) rp = DELMOD;
-                } else {
+            if constexpr (BU) {
+                // lanes 0 and 1 lock the two children at the same time
+                if (threadIdx.x < 2) {
+                    const int c = threadIdx.x == 0 ? l : r;
+                    const bool in = threadIdx.x == 0 ? lin : rin;
+                    int p = in ? lock_max(c) : AVAIL;
+                    if (is_inshold(p)) p = DELMOD;
+                    bc[BC_C + threadIdx.x] = p;
+                }
+                if (threadIdx.x == 0) bc[BC_E] = 2;
+            } else {
+                if (threadIdx.x == 0) {
+                    int type = 2;
                     // a child that is an insert target in flight is treated as
                     // absent (tditdd:425-446)
                     if (lin) {
@@ -346,37 +363,37 @@ struct DevHeap {
                         const int s = lock_child_td(r);
                         if (s != AVAIL) type = 1;
                     }
+                    bc[BC_C] = AVAIL;
+                    bc[BC_D] = AVAIL;
+                    bc[BC_E] = type;
                 }
-                bc[2] = lp;
-                bc[3] = rp;
-                bc[4] = type;
             }
-            sync_acq();
+            __syncthreads();
             if (aborted()) return;
-            const int lp = bc[2], rp = bc[3], type = bc[4];
+            const int lp = bc[BC_C], rp = bc[BC_D], type = bc[BC_E];
             if (type != 0) {
-                if (lin) copy_batch<K, B, NT>(X1, node(l));
-                else fill_batch<K, B, NT>(X1, h.init_limits);
+                if (lin) g2l_batch<K, B, NT>(X1, node(l));
+                else fill_lds<K, B, NT>(X1, h.init_limits);
             }
             if (type == 2) {
-                if (rin) copy_batch<K, B, NT>(X2, node(r));
-                else fill_batch<K, B, NT>(X2, h.init_limits);
+                if (rin) g2l_batch<K, B, NT>(X2, node(r));
+                else fill_lds<K, B, NT>(X2, h.init_limits);
             }
             __syncthreads();
 
             if constexpr (!BU) {
                 if (type == 0) {  // no child: the candidate stays here
-                    copy_batch<K, B, NT>(node(cur), X0);
-                    sync_rel();
-                    if (threadIdx.x == 0) a_store_rel(st(cur), AVAIL);
+                    l2g_batch<K, B, NT>(node(cur), X0);
+                    publish();
+                    if (threadIdx.x == 0) release_st(cur, AVAIL);
                     return;
                 }
                 if (type == 1) {  // only the left child, which is a leaf
-                    wg_merge<K, B, NT>(X0, X1, ToPtr<K>{node(cur)}, ToPtr<K>{node(l)});
-                    sync_rel();
+                    wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(cur)}, ToGlobal<K>{node(l)});
+                    publish();
                     if (threadIdx.x == 0) {
-                        a_store_rel(st(cur), AVAIL);
-                        a_store_rel(st(l), AVAIL);
+                        release_st(cur, AVAIL);
+                        release_st(l, AVAIL);
                     }
                     return;
                 }
@@ -389,30 +406,32 @@ struct DevHeap {
             const int tgt = tr ? r : l, oth = tr ? l : r;
             const bool tgt_in = tr ? rin : lin, oth_in = tr ? lin : rin;
             const int tgt_prev = tr ? rp : lp, oth_prev = tr ? lp : rp;
-            if (tgt_in) wg_merge<K, B, NT>(X1, X2, ToPtr<K>{X1}, ToPtr<K>{node(tgt)});
-            else wg_merge<K, B, NT>(X1, X2, ToPtr<K>{X1}, ToNone{});
-            sync_rel();
-            if (threadIdx.x == 0 && tgt_in) a_store_rel(st(tgt), BU ? tgt_prev : AVAIL);
+            if (tgt_in) wg_merge<K, B, NT>(X1, X2, ToLds<K>{X1}, ToGlobal<K>{node(tgt)});
+            else wg_merge<K, B, NT>(X1, X2, ToLds<K>{X1}, ToNone{});
 
             const K cmin = X0[0], cmax = X0[B - 1], kmin_ = X1[0], kmax_ = X1[B - 1];
             const bool stop = (VAR == VAR_BUITDD) ? (cmax < kmin_) : !(kmin_ < cmax);
             if (!(cmin < kmax_)) {
                 // candidate >= all of the children's small half: the node takes it
-                copy_batch<K, B, NT>(node(cur), X1);
+                l2g_batch<K, B, NT>(node(cur), X1);
             } else if (stop) {
-                copy_batch<K, B, NT>(node(cur), X0);
-                if (oth_in) copy_batch<K, B, NT>(node(oth), X1);
-                sync_rel();
+                l2g_batch<K, B, NT>(node(cur), X0);
+                if (oth_in) l2g_batch<K, B, NT>(node(oth), X1);
+                publish();
                 if (threadIdx.x == 0) {
-                    a_store_rel(st(cur), BU ? cur_prev : AVAIL);
-                    if (oth_in) a_store_rel(st(oth), BU ? oth_prev : AVAIL);
+                    if (tgt_in) release_st(tgt, BU ? tgt_prev : AVAIL);
+                    release_st(cur, BU ? cur_prev : AVAIL);
+                    if (oth_in) release_st(oth, BU ? oth_prev : AVAIL);
                 }
                 return;
             } else {
-                wg_merge<K, B, NT>(X0, X1, ToPtr<K>{node(cur)}, ToPtr<K>{X0});
+                wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(cur)}, ToLds<K>{X0});
             }
-            sync_rel();
-            if (threadIdx.x == 0) a_store_rel(st(cur), BU ? cur_prev : AVAIL);
+            publish();
+            if (threadIdx.x == 0) {
+                if (tgt_in) release_st(tgt, BU ? tgt_prev : AVAIL);
+                release_st(cur, BU ? cur_prev : AVAIL);
+            }
             if (!oth_in) {
                 // only reachable when keys equal init_limits (outside the contract)
                 flag_error(ERR_LOST_KEYS);
@@ -438,108 +457,109 @@ struct DevHeap {
 
         if (threadIdx.x == 0) {
             if (lock_cas(0, AVAIL, INUSE)) {
-                fence_acq();
-                bc[0] = a_load(ctr(CTR_BATCH_COUNT));
-                bc[1] = a_load(ctr(CTR_PARTIAL));
+                bc[BC_A] = a_load(ctr(CTR_BATCH_COUNT));
+                bc[BC_B] = a_load(ctr(CTR_PARTIAL));
                 if constexpr (BU) {
                     const int ticket = a_load(ctr(CTR_TICKET));
                     a_store(ctr(CTR_TICKET), ticket + 1);
-                    bc[6] = ((ticket & kTagMask) << 2) | INSHOLD;
+                    bc[BC_TAG] = ((ticket & kTagMask) << 2) | INSHOLD;
                 }
             }
         }
-        sync_acq();
+        __syncthreads();
         if (aborted()) return;
-        const int nb = bc[0], pbs = bc[1];
-        const int mine = BU ? bc[6] : 0;  // this insertion's INSHOLD word
+        const int nb = bc[BC_A], pbs = bc[BC_B];
+        const int mine = BU ? bc[BC_TAG] : 0;  // this insertion's INSHOLD word
         __syncthreads();
 
-        // ---- partial-buffer cases 1.1-2.2 (buitdd:522-613, tditdd:571-663)
+        // ---- partial-buffer cases 1.1-2.2 (buitdd:522-613, tditdd:571-663).
+        // The partial buffer and every counter only change under status[0],
+        // so they are read and written with plain coherent accesses here.
         if constexpr (PB) {
             if (nb == 0 && size < B) {
                 if (pbs == 0) {  // 1.1: the batch becomes the partial buffer
-                    copy_batch<K, B, NT>(node(0), X0);
-                    sync_rel();
+                    l2g_batch<K, B, NT>(node(0), X0);
+                    publish();
                     if (threadIdx.x == 0) {
                         a_store(ctr(CTR_PARTIAL), size);
-                        a_store_rel(st(0), AVAIL);
+                        release_st(0, AVAIL);
                     }
                     return;
                 }
-                copy_batch<K, B, NT>(X1, node(0));
+                g2l_batch<K, B, NT>(X1, node(0));
                 __syncthreads();
                 if (size + pbs < B) {  // 1.2: still no full batch
-                    wg_merge<K, B, NT>(X0, X1, ToPtr<K>{node(0)}, ToNone{});
-                    sync_rel();
+                    wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(0)}, ToNone{});
+                    publish();
                     if (threadIdx.x == 0) {
                         a_store(ctr(CTR_PARTIAL), pbs + size);
-                        a_store_rel(st(0), AVAIL);
+                        release_st(0, AVAIL);
                     }
                     return;
                 }
                 // 1.3: the smaller B keys become the root
                 if (threadIdx.x == 0) {
-                    a_add(ctr(CTR_BATCH_COUNT), 1);
+                    a_store(ctr(CTR_BATCH_COUNT), 1);
                     lock_cas(1, AVAIL, INUSE);
                 }
-                sync_acq();
+                __syncthreads();
                 if (aborted()) return;
-                wg_merge<K, B, NT>(X0, X1, ToPtr<K>{node(1)}, ToPtr<K>{node(0)});
-                sync_rel();
+                wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(1)}, ToGlobal<K>{node(0)});
+                publish();
                 if (threadIdx.x == 0) {
                     a_store(ctr(CTR_PARTIAL), pbs + size - B);
-                    a_store_rel(st(0), AVAIL);
-                    a_store_rel(st(1), AVAIL);
+                    release_st(1, AVAIL);
+                    release_st(0, AVAIL);
                 }
                 return;
             }
-            copy_batch<K, B, NT>(X1, node(0));
+            g2l_batch<K, B, NT>(X1, node(0));
             __syncthreads();
             if (size + pbs < B) {  // 2.1: fold into the partial buffer via the root
-                wg_merge<K, B, NT>(X0, X1, ToPtr<K>{X0}, ToNone{});
+                wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToNone{});
                 if (threadIdx.x == 0) lock_cas(1, AVAIL, INUSE);
-                sync_acq();
-                if (aborted()) return;
-                copy_batch<K, B, NT>(X1, node(1));
                 __syncthreads();
-                wg_merge<K, B, NT>(X0, X1, ToPtr<K>{node(1)}, ToPtr<K>{node(0)});
-                sync_rel();
+                if (aborted()) return;
+                g2l_batch<K, B, NT>(X1, node(1));
+                __syncthreads();
+                wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(1)}, ToGlobal<K>{node(0)});
+                publish();
                 if (threadIdx.x == 0) {
                     a_store(ctr(CTR_PARTIAL), pbs + size);
-                    a_store_rel(st(0), AVAIL);
-                    a_store_rel(st(1), AVAIL);
+                    release_st(1, AVAIL);
+                    release_st(0, AVAIL);
                 }
                 return;
             }
             // 2.2: a full batch goes into the tree, the rest stays partial
-            wg_merge<K, B, NT>(X0, X1, ToPtr<K>{X0}, ToPtr<K>{node(0)});
+            wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
             if (threadIdx.x == 0) a_store(ctr(CTR_PARTIAL), pbs + size - B);
+            publish();
         }
-        sync_rel();
 
         if constexpr (BU) {
             // ---- claim the next slot and sift up (buitdd:616-689)
             if (threadIdx.x == 0) {
-                const int idx = a_add(ctr(CTR_BATCH_COUNT), 1) + 1;
+                const int idx = nb + 1;
                 if (idx > h.batch_num) {
-                    a_add(ctr(CTR_BATCH_COUNT), -1);  // no slot: roll the claim back
-                    bc[2] = -1;
+                    bc[BC_C] = -1;
                 } else {
-                    bc[2] = lock_cas(idx, AVAIL, INUSE) ? idx : -2;
+                    a_store(ctr(CTR_BATCH_COUNT), idx);
+                    bc[BC_C] = lock_cas(idx, AVAIL, INUSE) ? idx : -2;
                 }
-                a_store_rel(st(0), AVAIL);
+                release_st(0, AVAIL);
             }
-            sync_acq();
+            __syncthreads();
             if (aborted()) return;
-            int cur = bc[2];
+            int cur = bc[BC_C];
             __syncthreads();
             if (cur < 0) {
                 flag_error(ERR_OVERFLOW);
                 return;
             }
-            copy_batch<K, B, NT>(node(cur), X0);
-            sync_rel();
-            if (threadIdx.x == 0) a_store_rel(st(cur), cur == 1 ? AVAIL : mine);
+            l2g_batch<K, B, NT>(node(cur), X0);
+            publish();
+            if (threadIdx.x == 0) release_st(cur, cur == 1 ? AVAIL : mine);
             // A concurrent deleter may pull keys of this insertion above the
             // node it sits in (it marks the node DELMOD, or takes it as its
             // last batch).  Such keys are only repaired by walking every level
@@ -553,29 +573,29 @@ struct DevHeap {
                 if (threadIdx.x == 0) {
                     int prev = -1;
                     if (lock_cas(par, AVAIL, INUSE)) prev = lock_max(cur);
-                    bc[3] = prev;
+                    bc[BC_D] = prev;
                 }
-                sync_acq();
+                __syncthreads();
                 if (aborted()) return;
-                dirty = dirty || (bc[3] != mine);
-                copy_batch<K, B, NT>(X0, node(cur));
-                copy_batch<K, B, NT>(X1, node(par));
+                dirty = dirty || (bc[BC_D] != mine);
+                g2l_batch<K, B, NT>(X0, node(cur));
+                g2l_batch<K, B, NT>(X1, node(par));
                 __syncthreads();
                 const bool identity = !(X0[0] < X1[B - 1]);  // parent.max <= node.min
                 const bool done = identity && !dirty;
                 if (!identity) {
                     if (!(X1[0] < X0[B - 1])) {
                         // parent.min >= node.max: swap (buitdd:647-661)
-                        copy_batch<K, B, NT>(node(par), X0);
-                        copy_batch<K, B, NT>(node(cur), X1);
+                        l2g_batch<K, B, NT>(node(par), X0);
+                        l2g_batch<K, B, NT>(node(cur), X1);
                     } else {
-                        wg_merge<K, B, NT>(X0, X1, ToPtr<K>{node(par)}, ToPtr<K>{node(cur)});
+                        wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(par)}, ToGlobal<K>{node(cur)});
                     }
                 }
-                sync_rel();
+                publish();
                 if (threadIdx.x == 0) {
-                    a_store_rel(st(par), (done || par == 1) ? AVAIL : mine);
-                    a_store_rel(st(cur), AVAIL);
+                    release_st(par, (done || par == 1) ? AVAIL : mine);
+                    release_st(cur, AVAIL);
                 }
                 if (done) return;
                 cur = par;
@@ -583,19 +603,19 @@ struct DevHeap {
         } else {
             // ---- claim a target slot and walk root -> target (tditdd:666-752)
             if (threadIdx.x == 0) {
-                const int tgt = a_add(ctr(CTR_BATCH_COUNT), 1) + 1;
+                const int tgt = nb + 1;
                 if (tgt > h.batch_num) {
-                    a_add(ctr(CTR_BATCH_COUNT), -1);  // no slot: roll the claim back
-                    bc[2] = -1;
-                    a_store_rel(st(0), AVAIL);
+                    bc[BC_C] = -1;
+                    release_st(0, AVAIL);
                 } else {
-                    bc[2] = tgt;
+                    a_store(ctr(CTR_BATCH_COUNT), tgt);
+                    bc[BC_C] = tgt;
                     if (lock_cas(tgt, AVAIL, TARGET) && tgt != 1) lock_cas(1, AVAIL, INUSE);
                 }
             }
-            sync_acq();
+            __syncthreads();
             if (aborted()) return;
-            const int tgt = bc[2];
+            const int tgt = bc[BC_C];
             __syncthreads();
             if (tgt < 0) {
                 flag_error(ERR_OVERFLOW);
@@ -604,56 +624,59 @@ struct DevHeap {
             int cur = 1;
             bool stolen = false;
             while (cur != tgt) {
-                if (threadIdx.x == 0) bc[3] = (a_load(st(tgt)) == MARKED);
+                if (threadIdx.x == 0) {
+                    // release the node above (for the root this is status[0]);
+                    // its data was published at the end of the previous step
+                    const int m = (a_load(st(tgt)) == MARKED);
+                    if (!m) release_st(cur >> 1, AVAIL);
+                    bc[BC_D] = m;
+                }
                 __syncthreads();
-                stolen = bc[3] != 0;
+                stolen = bc[BC_D] != 0;
                 __syncthreads();
                 if (stolen) break;
-                // release the node above (for the root this is status[0])
-                sync_rel();
-                if (threadIdx.x == 0) a_store_rel(st(cur >> 1), AVAIL);
-                copy_batch<K, B, NT>(X1, node(cur));
+                g2l_batch<K, B, NT>(X1, node(cur));
                 __syncthreads();
                 if (!(X1[0] < X0[B - 1])) {
                     // carried.max <= node.min: node takes the carried batch
-                    copy_batch<K, B, NT>(node(cur), X0);
-                    copy_batch<K, B, NT>(X0, X1);
+                    l2g_batch<K, B, NT>(node(cur), X0);
+                    l2l_batch<K, B, NT>(X0, X1);
                 } else if (X0[0] < X1[B - 1]) {
-                    wg_merge<K, B, NT>(X0, X1, ToPtr<K>{node(cur)}, ToPtr<K>{X0});
+                    wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(cur)}, ToLds<K>{X0});
                 }
-                __syncthreads();
+                publish();
                 const int nxt = next_toward(cur, tgt);
                 if (threadIdx.x == 0 && nxt != tgt) lock_cas(nxt, AVAIL, INUSE);
-                sync_acq();
+                __syncthreads();
                 if (aborted()) return;
                 cur = nxt;
             }
             if (!stolen) {
                 if (threadIdx.x == 0) {
                     a_cas(st(tgt), TARGET, INUSE);
-                    bc[3] = (a_load(st(tgt)) == MARKED);
+                    bc[BC_D] = (a_load(st(tgt)) == MARKED);
                 }
-                sync_acq();
-                stolen = bc[3] != 0;
+                __syncthreads();
+                stolen = bc[BC_D] != 0;
                 __syncthreads();
             }
             if (stolen) {
                 // a deleter took our target as its last batch: hand the carried
                 // batch over through the root slot, which that deleter holds
-                copy_batch<K, B, NT>(node(1), X0);
-                sync_rel();
+                l2g_batch<K, B, NT>(node(1), X0);
+                publish();
                 if (threadIdx.x == 0) {
-                    a_store_rel(st(cur >> 1), AVAIL);
-                    if (tgt != cur) a_store_rel(st(cur), AVAIL);
+                    release_st(cur >> 1, AVAIL);
+                    if (tgt != cur) release_st(cur, AVAIL);
                     lock_cas(tgt, MARKED, TARGET);
                 }
                 return;
             }
-            copy_batch<K, B, NT>(node(tgt), X0);
-            sync_rel();
+            l2g_batch<K, B, NT>(node(tgt), X0);
+            publish();
             if (threadIdx.x == 0) {
-                a_store_rel(st(cur >> 1), AVAIL);
-                a_store_rel(st(cur), AVAIL);
+                release_st(cur >> 1, AVAIL);
+                release_st(cur, AVAIL);
             }
         }
     }
@@ -664,15 +687,14 @@ struct DevHeap {
         int* bc = bcast_slots();
         if (!op_begin()) return 0;
         if (threadIdx.x == 0) {
-            bc[5] = 0;
+            bc[BC_F] = 0;
             if (lock_cas(0, AVAIL, INUSE)) {
-                fence_acq();
-                bc[5] = a_load(ctr(CTR_PARTIAL)) + a_load(ctr(CTR_BATCH_COUNT)) * B;
-                a_store_rel(st(0), AVAIL);
+                bc[BC_F] = a_load(ctr(CTR_PARTIAL)) + a_load(ctr(CTR_BATCH_COUNT)) * B;
+                release_st(0, AVAIL);
             }
         }
         __syncthreads();
-        const int n = bc[5];
+        const int n = bc[BC_F];
         __syncthreads();
         return n;
     }

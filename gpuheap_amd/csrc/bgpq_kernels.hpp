@@ -65,20 +65,22 @@ __global__ __launch_bounds__(threads_for(B)) void k_prim_sort(const K* in, const
     extern __shared__ __align__(16) int lds[];
     K* X0 = reinterpret_cast<K*>(lds);
     wg_sort_batch<K, B, NT>(X0, in + (size_t)blockIdx.x * B, sizes[blockIdx.x], pad);
-    copy_batch<K, B, NT>(out + (size_t)blockIdx.x * B, X0);
+    l2g_batch<K, B, NT>(out + (size_t)blockIdx.x * B, X0);
 }
 
-template <typename K, int B>
+// algo 0: the register bitonic merge, 1: the LDS merge path
+template <typename K, int B, int ALGO>
 __global__ __launch_bounds__(threads_for(B)) void k_prim_merge(const K* a, const K* b, K* small, K* large) {
     constexpr int NT = threads_for(B);
     extern __shared__ __align__(16) int lds[];
     K* X0 = reinterpret_cast<K*>(lds);
     K* X1 = X0 + B;
     const size_t o = (size_t)blockIdx.x * B;
-    copy_batch<K, B, NT>(X0, a + o);
-    copy_batch<K, B, NT>(X1, b + o);
+    g2l_batch<K, B, NT>(X0, a + o);
+    g2l_batch<K, B, NT>(X1, b + o);
     __syncthreads();
-    wg_merge<K, B, NT>(X0, X1, ToPtr<K>{small + o}, ToPtr<K>{large + o});
+    if constexpr (ALGO == 0) wg_merge_bitonic<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
+    else wg_merge_mp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
 }
 
 template <typename K>
@@ -101,7 +103,7 @@ struct HeapArgs {
 struct LaunchRow {
     int batch_size;
     hipError_t (*sort)(const void* in, const int* sizes, void* out, uint64_t pad, int batches, hipStream_t s);
-    hipError_t (*merge)(const void* a, const void* b, void* small, void* large, int batches, hipStream_t s);
+    hipError_t (*merge)(const void* a, const void* b, void* small, void* large, int batches, int algo, hipStream_t s);
     hipError_t (*insert)(const HeapArgs&, const void* keys, long long n, int chunk, int grid, hipStream_t s);
     hipError_t (*del)(const HeapArgs&, void* out, long long calls, int grid, hipStream_t s);
     hipError_t (*conc)(const HeapArgs&, const void* keys, long long n, int chunk, void* out, long long calls,
@@ -116,6 +118,7 @@ inline HeapView<K> view_of(const HeapArgs& a) {
 template <typename K, int VAR, int B>
 struct Launch {
     static constexpr size_t lds = (size_t)delete_lds_bytes<K, B>();
+    static_assert(delete_lds_bytes<K, B>() <= 64 * 1024, "dynamic LDS per workgroup");
     static hipError_t insert(const HeapArgs& a, const void* keys, long long n, int chunk, int grid,
                              hipStream_t s) {
         hipLaunchKernelGGL((k_insert<K, VAR, B>), dim3(grid), dim3(threads_for(B)), lds, s, view_of<K>(a),
@@ -138,9 +141,14 @@ struct Launch {
                            sizes, (K*)out, (K)pad);
         return hipGetLastError();
     }
-    static hipError_t merge(const void* a, const void* b, void* small, void* large, int batches, hipStream_t s) {
-        hipLaunchKernelGGL((k_prim_merge<K, B>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
-                           (const K*)a, (const K*)b, (K*)small, (K*)large);
+    static hipError_t merge(const void* a, const void* b, void* small, void* large, int batches, int algo,
+                            hipStream_t s) {
+        if (algo == 0)
+            hipLaunchKernelGGL((k_prim_merge<K, B, 0>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
+                               (const K*)a, (const K*)b, (K*)small, (K*)large);
+        else
+            hipLaunchKernelGGL((k_prim_merge<K, B, 1>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
+                               (const K*)a, (const K*)b, (K*)small, (K*)large);
         return hipGetLastError();
     }
     static LaunchRow row() { return LaunchRow{B, &sort, &merge, &insert, &del, &conc}; }

@@ -2,9 +2,9 @@
 // written for CDNA4 (gfx950, wave64).
 //
 // Counterpart of the reference's heap/heaputil.cuh:
-//   batchCopy   (heap/heaputil.cuh:4-12)   -> load_batch / store_batch / fill_batch
+//   batchCopy   (heap/heaputil.cuh:4-12)   -> g2l_batch / l2g_batch / take_batch
 //   ibitonicSort(heap/heaputil.cuh:23-47)  -> wg_sort_batch
-//   imergePath  (heap/heaputil.cuh:99-158) -> wg_merge_path (+ the store helpers)
+//   imergePath  (heap/heaputil.cuh:99-158) -> wg_merge_path / wg_merge_bitonic
 //
 // Layout conventions
 //   * A batch is B keys, ascending, B a power of two.
@@ -20,18 +20,50 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace bgpq {
 
 // ---------------------------------------------------------------- memory model
-// Cross-workgroup hand-off on gfx950: a workgroup that publishes batch data
-// runs an agent-scope release fence in every wave (buffer_wbl2 sc1 +
-// s_waitcnt vmcnt(0)), then a barrier, then lane 0 stores the status word.
-// The consumer's lane 0 spins on the status word, runs an acquire fence,
-// then a barrier, then every wave runs its own acquire fence (buffer_inv sc1)
-// before reading batch data.  __syncthreads() alone does not wait for other
-// waves' global stores on gfx950, hence the per-wave fences.
-__device__ __forceinline__ void fence_rel() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
-__device__ __forceinline__ void fence_acq() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+// Cross-workgroup hand-off on gfx950.  Every access to shared heap data is
+// device-coherent: 16-byte buffer loads and stores with the sc1 cache policy
+// (the encoding LLVM uses for agent-scope atomic loads and stores), which
+// neither hit stale lines in another XCD's L2 nor leave dirty lines behind.
+// A publisher therefore only has to wait until its own stores are acknowledged
+// (s_waitcnt vmcnt(0) in every wave), pass a barrier, and let lane 0 store the
+// status word; a consumer's lane 0 spins on the status word, then a barrier
+// orders every wave's data loads after it.  No whole-L2 writeback
+// (buffer_wbl2) or invalidate (buffer_inv) is needed, which is what the plain
+// acquire/release fences cost (about 0.7 us and 0.3 us per wave on MI355X).
+constexpr int kSC1 = 16;  // CPol::SC1 on gfx940+: device-coherent
+
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base) {
+    // raw buffer over [base, base + 2 GiB); 0x00020000 = the gfx9 word-3 format
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, 0x7FFFFFFF, 0x00020000);
+}
+__device__ __forceinline__ uint4 gload16(__amdgpu_buffer_rsrc_t r, int byte_off) {
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, kSC1);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gstore16(__amdgpu_buffer_rsrc_t r, int byte_off, uint4 q) {
+    u32x4 v;
+    v.x = q.x; v.y = q.y; v.z = q.z; v.w = q.w;
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, byte_off, 0, kSC1);
+}
+
+// single-key coherent access (relaxed agent-scope atomics compile to sc1)
+template <typename K>
+__device__ __forceinline__ K gload_key(const K* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename K>
+__device__ __forceinline__ void gstore_key(K* p, K v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ int a_load(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -59,6 +91,8 @@ __device__ __forceinline__ int a_max(int* p, int v) {
 }
 __device__ __forceinline__ void backoff() { __builtin_amdgcn_s_sleep(1); }
 
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
+
 // ---------------------------------------------------------------- key helpers
 template <typename K> __device__ __forceinline__ K kmin(K a, K b) { return b < a ? b : a; }
 template <typename K> __device__ __forceinline__ K kmax(K a, K b) { return b < a ? a : b; }
@@ -85,43 +119,95 @@ template <int M> __device__ __forceinline__ uint64_t xlane(uint64_t v) {
 }
 
 // ---------------------------------------------------------------- vector moves
-// Copy B keys global -> LDS or LDS -> global with 16-byte lanes.
+// Heap nodes live in global memory and are moved with coherent 16-byte
+// accesses; LDS copies are plain.  Thread t moves chunks c = t, t+NT, ...
 template <typename K, int B, int NT>
-__device__ __forceinline__ void copy_batch(K* __restrict__ dst, const K* __restrict__ src) {
+__device__ __forceinline__ void g2l_batch(K* __restrict__ lds, const K* __restrict__ g) {
+    constexpr int NV = B * (int)sizeof(K) / 16;
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(g);
+    uint4* d = reinterpret_cast<uint4*>(lds);
+#pragma unroll
+    for (int k = 0; k < (NV + NT - 1) / NT; ++k) {
+        const int c = threadIdx.x + k * NT;
+        if (NV % NT == 0 || c < NV) d[c] = gload16(r, c * 16);
+    }
+}
+template <typename K, int B, int NT>
+__device__ __forceinline__ void l2g_batch(K* __restrict__ g, const K* __restrict__ lds) {
+    constexpr int NV = B * (int)sizeof(K) / 16;
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(g);
+    const uint4* s = reinterpret_cast<const uint4*>(lds);
+#pragma unroll
+    for (int k = 0; k < (NV + NT - 1) / NT; ++k) {
+        const int c = threadIdx.x + k * NT;
+        if (NV % NT == 0 || c < NV) gstore16(r, c * 16, s[c]);
+    }
+}
+template <typename K, int B, int NT>
+__device__ __forceinline__ void l2l_batch(K* __restrict__ dst, const K* __restrict__ src) {
     constexpr int NV = B * (int)sizeof(K) / 16;
     const uint4* s = reinterpret_cast<const uint4*>(src);
     uint4* d = reinterpret_cast<uint4*>(dst);
 #pragma unroll
-    for (int c = threadIdx.x; c < NV; c += NT) d[c] = s[c];
+    for (int k = 0; k < (NV + NT - 1) / NT; ++k) {
+        const int c = threadIdx.x + k * NT;
+        if (NV % NT == 0 || c < NV) d[c] = s[c];
+    }
+}
+// global -> global (heap node to caller buffer), coherent loads
+template <typename K, int B, int NT>
+__device__ __forceinline__ void g2g_batch(K* __restrict__ dst, const K* __restrict__ g) {
+    constexpr int NV = B * (int)sizeof(K) / 16;
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(g);
+    uint4* d = reinterpret_cast<uint4*>(dst);
+#pragma unroll
+    for (int k = 0; k < (NV + NT - 1) / NT; ++k) {
+        const int c = threadIdx.x + k * NT;
+        if (NV % NT == 0 || c < NV) d[c] = gload16(r, c * 16);
+    }
+}
+
+template <typename K>
+__device__ __forceinline__ uint4 splat16(K v) {
+    uint4 f;
+    if constexpr (sizeof(K) == 4) {
+        f.x = f.y = f.z = f.w = (uint32_t)v;
+    } else {
+        f.x = f.z = (uint32_t)v;
+        f.y = f.w = (uint32_t)((uint64_t)v >> 32);
+    }
+    return f;
 }
 
 // Move B keys global -> LDS and reset the source to `fill` in the same
 // pass: each thread resets exactly the 16-byte chunk it loaded, so no other
 // thread can overwrite a chunk before it is read.
 template <typename K, int B, int NT>
-__device__ __forceinline__ void take_batch(K* __restrict__ dst, K* __restrict__ src, K fill) {
+__device__ __forceinline__ void take_batch(K* __restrict__ lds, K* __restrict__ g, K fill) {
     constexpr int NV = B * (int)sizeof(K) / 16;
-    uint4 f;
-    if constexpr (sizeof(K) == 4) {
-        f.x = f.y = f.z = f.w = (uint32_t)fill;
-    } else {
-        f.x = f.z = (uint32_t)fill;
-        f.y = f.w = (uint32_t)((uint64_t)fill >> 32);
-    }
-    uint4* s = reinterpret_cast<uint4*>(src);
-    uint4* d = reinterpret_cast<uint4*>(dst);
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(g);
+    const uint4 f = splat16(fill);
+    uint4* d = reinterpret_cast<uint4*>(lds);
 #pragma unroll
-    for (int c = threadIdx.x; c < NV; c += NT) {
-        const uint4 v = s[c];
-        d[c] = v;
-        s[c] = f;
+    for (int k = 0; k < (NV + NT - 1) / NT; ++k) {
+        const int c = threadIdx.x + k * NT;
+        if (NV % NT == 0 || c < NV) {
+            d[c] = gload16(r, c * 16);
+            gstore16(r, c * 16, f);
+        }
     }
 }
 
 template <typename K, int B, int NT>
-__device__ __forceinline__ void fill_batch(K* dst, K v) {
+__device__ __forceinline__ void fill_lds(K* lds, K v) {
+    constexpr int NV = B * (int)sizeof(K) / 16;
+    const uint4 f = splat16(v);
+    uint4* d = reinterpret_cast<uint4*>(lds);
 #pragma unroll
-    for (int i = threadIdx.x; i < B; i += NT) dst[i] = v;
+    for (int k = 0; k < (NV + NT - 1) / NT; ++k) {
+        const int c = threadIdx.x + k * NT;
+        if (NV % NT == 0 || c < NV) d[c] = f;
+    }
 }
 
 // Store a thread's run of N consecutive keys (N*sizeof(K) a multiple of 16
@@ -281,31 +367,150 @@ __device__ __forceinline__ void wg_merge_path(const K* a, const K* b, K (&out)[2
     }
 }
 
-// Destination tags for the two halves of a merge.
+// Destination tags for the halves of a merge: LDS, global heap node
+// (coherent stores), or dropped.
 struct ToNone {};
-template <typename K> struct ToPtr { K* p; };   // LDS or global, B keys
+template <typename K> struct ToLds { K* p; };
+template <typename K> struct ToGlobal { K* p; };
 
-template <typename K, int B, int NT, typename D>
-__device__ __forceinline__ void put_half(const D& dst, const K (&out)[2 * B / NT], int run_off) {
-    if constexpr (!std::is_same<D, ToNone>::value) {
-        constexpr int L = 2 * B / NT;
-        store_run<K, L>(dst.p + run_off, out);
+// Store a thread's run of N keys at key offset `off` of a destination.
+template <typename K, int N, typename D>
+__device__ __forceinline__ void put_run(const D& dst, int off, const K (&v)[N]) {
+    if constexpr (std::is_same<D, ToNone>::value) {
+        return;
+    } else if constexpr (std::is_same<D, ToLds<K>>::value) {
+        store_run<K, N>(dst.p + off, v);
+    } else if constexpr ((N * sizeof(K)) % 16 != 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) gstore_key(dst.p + off + i, v[i]);
+    } else {
+        const __amdgpu_buffer_rsrc_t r = buf_rsrc(dst.p);
+        constexpr int W = 16 / (int)sizeof(K);
+#pragma unroll
+        for (int c = 0; c < N / W; ++c) {
+            uint4 q;
+            if constexpr (sizeof(K) == 4) {
+                q.x = (uint32_t)v[c * 4 + 0]; q.y = (uint32_t)v[c * 4 + 1];
+                q.z = (uint32_t)v[c * 4 + 2]; q.w = (uint32_t)v[c * 4 + 3];
+            } else {
+                q.x = (uint32_t)v[c * 2 + 0]; q.y = (uint32_t)((uint64_t)v[c * 2 + 0] >> 32);
+                q.z = (uint32_t)v[c * 2 + 1]; q.w = (uint32_t)((uint64_t)v[c * 2 + 1] >> 32);
+            }
+            gstore16(r, (off + c * W) * (int)sizeof(K), q);
+        }
     }
 }
 
-// Merge a and b (LDS) and write the B smallest keys to `small`, the B largest
-// to `large`.  Destinations may alias a or b: all reads finish before the
-// first write.  Ends with a barrier.
+// Merge a and b (LDS) with the merge-path kernel above and write the B
+// smallest keys to `small`, the B largest to `large`.  Destinations may
+// alias a or b.  Ends with a barrier.
 template <typename K, int B, int NT, typename DS, typename DL>
-__device__ __forceinline__ void wg_merge(const K* a, const K* b, const DS& small, const DL& large) {
+__device__ __forceinline__ void wg_merge_mp(const K* a, const K* b, const DS& small, const DL& large) {
     constexpr int L = 2 * B / NT;
     K out[L];
     wg_merge_path<K, B, NT>(a, b, out);
     __syncthreads();
     const int d = threadIdx.x * L;
-    if (d < B) put_half<K, B, NT>(small, out, d);
-    else put_half<K, B, NT>(large, out, d - B);
+    if (d < B) put_run<K, L>(small, d, out);
+    else put_run<K, L>(large, d - B, out);
     __syncthreads();
 }
+
+// Register bitonic merge.  For ascending a and b, c[i] = min(a[i], b[B-1-i])
+// holds the B smallest keys of a and b and d[i] = max(a[i], b[B-1-i]) the B
+// largest, each as a bitonic sequence (Batcher's half-cleaner on a ++
+// reverse(b)); log2(B) half-cleaning stages then sort c and d together.
+// Thread t keeps c[tP..tP+P) and d[tP..tP+P) in registers, P = B/NT: strides
+// below P stay in the lane, strides below 64P use DPP / ds_swizzle /
+// ds_bpermute, and larger ones go through LDS, reusing a and b as scratch.
+// No step depends on the key values, so the latency is fixed and the LDS
+// traffic is bank-conflict free.  Ends with a barrier.
+template <typename K, int P, int NT>
+__device__ __forceinline__ void xlane_minmax(K (&c)[P], K (&d)[P], int m, bool lower) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        K wc, wd;
+        switch (m) {
+            case 1: wc = xlane<1>(c[i]); wd = xlane<1>(d[i]); break;
+            case 2: wc = xlane<2>(c[i]); wd = xlane<2>(d[i]); break;
+            case 4: wc = xlane<4>(c[i]); wd = xlane<4>(d[i]); break;
+            case 8: wc = xlane<8>(c[i]); wd = xlane<8>(d[i]); break;
+            case 16: wc = xlane<16>(c[i]); wd = xlane<16>(d[i]); break;
+            default: wc = xlane<32>(c[i]); wd = xlane<32>(d[i]); break;
+        }
+        c[i] = lower ? kmin(c[i], wc) : kmax(c[i], wc);
+        d[i] = lower ? kmin(d[i], wd) : kmax(d[i], wd);
+    }
+}
+
+template <typename K, int B, int NT, typename DS, typename DL>
+__device__ __forceinline__ void wg_merge_bitonic(K* a, K* b, const DS& small, const DL& large) {
+    constexpr int P = B / NT;
+    const int t = threadIdx.x;
+    K c[P], d[P], av[P], bv[P];
+    load_run<K, P>(av, a + t * P);
+    load_run<K, P>(bv, b + (B - P - t * P));
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        const K x = av[i], y = bv[P - 1 - i];
+        c[i] = kmin(x, y);
+        d[i] = kmax(x, y);
+    }
+    constexpr bool kCrossWave = (B / 2) >= 64 * P;
+    if constexpr (kCrossWave) __syncthreads();  // a and b become exchange scratch
+    constexpr int kStages = ilog2(B);
+#pragma unroll
+    for (int s = 0; s < kStages; ++s) {
+        const int j = (B / 2) >> s;
+        if (j < P) {
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                if ((i & j) == 0) {
+                    const K c0 = c[i], c1 = c[i | j], d0 = d[i], d1 = d[i | j];
+                    c[i] = kmin(c0, c1); c[i | j] = kmax(c0, c1);
+                    d[i] = kmin(d0, d1); d[i | j] = kmax(d0, d1);
+                }
+            }
+        } else {
+            const int m = j / P;
+            const bool lower = (t & m) == 0;
+            if (j < 64 * P) {
+                xlane_minmax<K, P, NT>(c, d, m, lower);
+            } else {
+                store_run<K, P>(a + t * P, c);
+                store_run<K, P>(b + t * P, d);
+                __syncthreads();
+                K wc[P], wd[P];
+                load_run<K, P>(wc, a + (t ^ m) * P);
+                load_run<K, P>(wd, b + (t ^ m) * P);
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    c[i] = lower ? kmin(c[i], wc[i]) : kmax(c[i], wc[i]);
+                    d[i] = lower ? kmin(d[i], wd[i]) : kmax(d[i], wd[i]);
+                }
+            }
+        }
+    }
+    if constexpr (!kCrossWave) __syncthreads();  // all reads of a and b done
+    put_run<K, P>(small, t * P, c);
+    put_run<K, P>(large, t * P, d);
+    __syncthreads();
+}
+
+// The merge used by the heap.  Both kernels are kept: the merge-path one
+// (imergePath's algorithm) and the register bitonic one, which is faster on
+// MI355X because its latency does not depend on data-dependent LDS reads.
+#ifndef BGPQ_MERGE_PATH
+template <typename K, int B, int NT, typename DS, typename DL>
+__device__ __forceinline__ void wg_merge(K* a, K* b, const DS& small, const DL& large) {
+    wg_merge_bitonic<K, B, NT>(a, b, small, large);
+}
+#else
+template <typename K, int B, int NT, typename DS, typename DL>
+__device__ __forceinline__ void wg_merge(K* a, K* b, const DS& small, const DL& large) {
+    wg_merge_mp<K, B, NT>(a, b, small, large);
+}
+#endif
 
 }  // namespace bgpq

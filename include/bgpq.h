@@ -96,11 +96,12 @@ int bgpq_default_grid(int batch_size);
  * bgpq_test_sort: the insert path's pad-and-sort (ibitonicSort, heap/heaputil.cuh:23-47):
  *   batch i = d_in[i*B .. i*B+d_sizes[i]) padded with `pad`, sorted into d_out[i*B ..].
  * bgpq_test_merge: merge of sorted d_a[i], d_b[i] (imergePath, heap/heaputil.cuh:99-158):
- *   the B smallest keys to d_small[i], the B largest to d_large[i]. */
+ *   the B smallest keys to d_small[i], the B largest to d_large[i]; algo 0 is
+ *   the register bitonic merge the heap uses, 1 the LDS merge path. */
 int bgpq_test_sort(int key_type, int batch_size, const void* d_in, const int* d_sizes, void* d_out,
                    uint64_t pad, int batches);
 int bgpq_test_merge(int key_type, int batch_size, const void* d_a, const void* d_b, void* d_small, void* d_large,
-                    int batches);
+                    int batches, int algo);
 
 const char* bgpq_last_error(void);
 

@@ -1,7 +1,8 @@
 """The batch primitives on the GPU against numpy, for every built batch size:
 the insert path's pad-and-sort (ibitonicSort, heap/heaputil.cuh:23-47) must
-equal numpy's sort of the padded batch, and the merge (imergePath,
-heap/heaputil.cuh:99-158) must split the sorted union at B."""
+equal numpy's sort of the padded batch, and both merge kernels (imergePath,
+heap/heaputil.cuh:99-158: the register bitonic merge the heap uses and the
+LDS merge path) must split the sorted union at B."""
 import numpy as np
 import pytest
 
@@ -44,7 +45,8 @@ def test_sort_batch(gpu, B, key):
 
 @pytest.mark.parametrize("B", BATCHES)
 @pytest.mark.parametrize("key", ["u32", "u64"])
-def test_merge_batches(gpu, B, key):
+@pytest.mark.parametrize("algo", [0, 1], ids=["bitonic", "merge_path"])
+def test_merge_batches(gpu, B, key, algo):
     kt, dtype = KEYS[key]
     rng = np.random.default_rng(B + 1)
     nb = 64
@@ -57,7 +59,7 @@ def test_merge_batches(gpu, B, key):
     a[3], b[3] = b[2].copy(), a[2].copy()
     d = [DeviceBuffer.from_numpy(x) for x in (a, b)]
     small, large = DeviceBuffer(a.nbytes), DeviceBuffer(a.nbytes)
-    check(lib().bgpq_test_merge(kt, B, d[0].ptr, d[1].ptr, small.ptr, large.ptr, nb))
+    check(lib().bgpq_test_merge(kt, B, d[0].ptr, d[1].ptr, small.ptr, large.ptr, nb, algo))
     s = small.to_numpy(dtype).reshape(nb, B)
     l_ = large.to_numpy(dtype).reshape(nb, B)
     for i in range(nb):
