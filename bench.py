@@ -1,0 +1,227 @@
+"""Benchmark of the batched GPU priority queue on one MI355X.
+
+One step = one heap sort of N synthetic keys through the heap: insert all
+keys in batch-sized insertion() calls, then deleteRoot/deleteUpdate until the
+heap is empty (the reference harness, test/heap_test.cu:135-162).  The keys
+are generated on the device before timing and stay resident; every step reads
+them and writes the sorted output to a second buffer.
+
+Prints one JSON line (the driver's contract) with keys/s, the roofline of the
+dominant kernel and the CPU baseline (std::sort and std::priority_queue on a
+bounded sample, timed on this host).
+
+    python bench.py [--steps K] [--warmup W] [--n LOG2|N] [--variant V] [--batch B]
+    python bench.py --sweep            # the heap-sort sweep, one JSON line per config
+"""
+import argparse
+import ctypes
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+import gpuheap_amd as G  # noqa: E402
+from gpuheap_amd import DeviceBuffer, Heap, batch_num_for  # noqa: E402
+from gpuheap_amd._lib import check, lib  # noqa: E402
+
+METRIC = "keys/s insert+delete (2^28 u32) & %HBM peak; batched solves/s + p99 latency"
+HBM_PEAK_GBS = 8000.0
+KEYS = {"u32": (0, np.uint32, 4), "u64": (1, np.uint64, 8)}
+
+
+def levels_sum(m):
+    """S(M) = sum_{k=1..M} floor(log2 k): tree levels below the root summed
+    over the heap sizes 1..M batches."""
+    total, lvl, start = 0, 0, 1
+    while start <= m:
+        end = min(2 * start - 1, m)
+        total += lvl * (end - start + 1)
+        lvl += 1
+        start *= 2
+    return total
+
+
+def alg_bytes(n, b, s):
+    """Algorithmic HBM bytes of a heap sort of n keys, batch b, key size s:
+    insert reads the input once and reads+writes one batch per level; delete
+    writes the output once, reads two children and writes two nodes per level."""
+    m = n // b
+    sm = levels_sum(m)
+    return {"insert": n * s + 2 * b * s * sm, "delete": n * s + 4 * b * s * sm,
+            "total": 2 * n * s + 6 * b * s * sm}
+
+
+def parse_n(v):
+    v = int(v)
+    return 1 << v if v < 64 else v
+
+
+def fingerprint(a):
+    """Order-independent fingerprint of a key multiset (sum, xor, sum of
+    squares, all mod 2^64)."""
+    x = a.astype(np.uint64)
+    with np.errstate(over="ignore"):
+        return (int(np.sum(x, dtype=np.uint64)), int(np.bitwise_xor.reduce(x)),
+                int(np.sum(x * x, dtype=np.uint64)))
+
+
+def run_config(n, key, variant, batch, gi, gd, steps, warmup, order="random", seed=0, verify=True):
+    kt, dtype, ks = KEYS[key]
+    d_in = DeviceBuffer(n * ks)
+    d_out = DeviceBuffer(n * ks)
+    if order == "random":
+        check(lib().bgpq_gen_keys(kt, seed, n, d_in.ptr, None))
+    else:
+        check(lib().bgpq_gen_keys(kt, seed, n, d_in.ptr, None))
+        host = np.sort(d_in.to_numpy(dtype, n))
+        if order == "descending":
+            host = host[::-1].copy()
+        d_in.upload(host)
+        del host
+    G.sync()
+    h = Heap(batch_num_for(n, batch), batch, variant=variant, key=key)
+    for _ in range(warmup):
+        h.heapsort(d_in, n, grid_insert=gi, grid_delete=gd, d_out=d_out)
+    G.sync()
+    ins, dele = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a, b = h.heapsort(d_in, n, grid_insert=gi, grid_delete=gd, d_out=d_out)
+        ins.append(a)
+        dele.append(b)
+    G.sync()
+    wall = time.perf_counter() - t0
+    ok = None
+    if verify:
+        src = d_in.to_numpy(dtype, n)
+        out = d_out.to_numpy(dtype, n)
+        ok = bool(np.all(out[1:] >= out[:-1])) and fingerprint(src) == fingerprint(out)
+        del src, out
+    h.close()
+    d_in.free()
+    d_out.free()
+    t_ins, t_del = float(np.mean(ins)), float(np.mean(dele))
+    return {"ms_insert": t_ins, "ms_delete": t_del, "ms_step_wall": 1e3 * wall / max(steps, 1),
+            "keys_per_s": n / ((t_ins + t_del) / 1e3), "ok": ok}
+
+
+def cpu_baseline(key, sample_log2=26, pq_log2=24):
+    exe = os.path.join(REPO, "oracle", "_build", "cpu_baseline")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True, capture_output=True)
+    r = subprocess.run([exe, key, str(1 << sample_log2), "0", "1", str(1 << pq_log2)], capture_output=True,
+                       text=True, check=True)
+    j = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": j["std_sort_keys_per_s"], "unit": "keys/s", "cores": 1, "kind": "port",
+            "sample": (f"std::sort of 2^{sample_log2} SplitMix64 seed-0 {key} keys (1 warm-up, 1 timed), "
+                       f"single thread; std::priority_queue push-all/pop-all of 2^{pq_log2}: "
+                       f"{j['pq_keys_per_s']:.3e} keys/s"),
+            "pq_keys_per_s": j["pq_keys_per_s"], "ok": j["ok"], "host_threads_available": os.cpu_count()}
+
+
+def traffic_for(workload):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/hbm_traffic.json)."""
+    p = os.path.join(REPO, "profiles", "hbm_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=parse_n, default=1 << 28, help="keys (or log2 of keys if < 64)")
+    ap.add_argument("--key", choices=sorted(KEYS), default="u32")
+    ap.add_argument("--variant", choices=sorted(G.VARIANTS), default="buitdd_np")
+    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--grid-insert", type=int, default=256)
+    ap.add_argument("--grid-delete", type=int, default=32)
+    ap.add_argument("--order", choices=["random", "ascending", "descending"], default="random")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a plain 1 GiB device copy (HBM counter calibration under rocprofv3)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    print(f"[bench] device: {G.device_name()}", file=sys.stderr, flush=True)
+
+    if args.sweep:
+        for key, order in (("u32", "random"), ("u32", "descending"), ("u64", "random")):
+            for variant in ("buitdd", "buitdd_np", "tditdd", "tditdd_np"):
+                for b in (256, 512, 1024, 2048):
+                    r = run_config(args.n, key, variant, b, args.grid_insert, args.grid_delete, 1, 0, order=order)
+                    ab = alg_bytes(args.n, b, KEYS[key][2])
+                    r.update({"n": args.n, "key": key, "order": order, "variant": variant, "batch": b,
+                              "hbm_frac_total": ab["total"] / ((r["ms_insert"] + r["ms_delete"]) / 1e3) / 8e12})
+                    print(json.dumps(r), flush=True)
+        return
+
+    print(f"[bench] heap sort n={args.n} {args.key} {args.variant} B={args.batch} "
+          f"grids {args.grid_insert}/{args.grid_delete}", file=sys.stderr, flush=True)
+    r = run_config(args.n, args.key, args.variant, args.batch, args.grid_insert, args.grid_delete, args.steps,
+                   args.warmup, order=args.order)
+    if not r["ok"]:
+        print("[bench] output check FAILED", file=sys.stderr, flush=True)
+    s = KEYS[args.key][2]
+    ab = alg_bytes(args.n, args.batch, s)
+    dom = "delete" if r["ms_delete"] >= r["ms_insert"] else "insert"
+    dom_ms = r["ms_" + dom]
+    achieved = ab[dom] / (dom_ms / 1e3) / 1e9
+    workload = f"heapsort_{args.key}_n{args.n}_{args.variant}_b{args.batch}_{args.order}"
+    roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for(workload),
+                "alg_bytes_per_launch": ab[dom]}
+    line = {
+        "metric": METRIC,
+        "value": r["keys_per_s"],
+        "unit": "keys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_step_wall"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.key,
+        "data": f"synthetic: SplitMix64 seed 0 {args.key} keys, {args.order} order, generated on device",
+        "config": {"workload": f"heap sort (insert all, delete until empty) of {args.n} {args.key} keys",
+                   "n_keys": args.n, "key": args.key, "variant": args.variant, "batch_size": args.batch,
+                   "grid_insert": args.grid_insert, "grid_delete": args.grid_delete, "order": args.order},
+        "phases_ms": {"insert": r["ms_insert"], "delete": r["ms_delete"]},
+        "hbm_frac_total": ab["total"] / ((r["ms_insert"] + r["ms_delete"]) / 1e3) / (HBM_PEAK_GBS * 1e9),
+        "check": "output sorted and multiset fingerprint equal to input: " + ("ok" if r["ok"] else "FAILED"),
+        "roofline": roofline,
+    }
+    if args.calibrate:
+        nb = 1 << 30
+        src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
+        check(lib().bgpq_dev_copy(dst.ptr, src.ptr, nb, None))
+        G.sync()
+        line["calibration_copy_bytes"] = nb
+    if not args.no_cpu:
+        print("[bench] CPU baseline (std::sort, std::priority_queue) ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(args.key)
+    print(json.dumps(line), flush=True)
+    if not r["ok"]:
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

@@ -33,7 +33,7 @@ SIGNATURES = {
     "bgpq_insert": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_int, c_void_p]),
     "bgpq_delete": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_void_p]),
     "bgpq_concurrent": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_void_p, c_i64, c_int, c_void_p]),
-    "bgpq_heapsort": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_int, c_int, c_float_p, c_float_p, c_void_p]),
+    "bgpq_heapsort": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int, c_int, c_float_p, c_float_p, c_void_p]),
     "bgpq_snapshot": (c_int, [c_void_p, c_void_p, c_int_p, c_int_p]),
     "bgpq_error_flags": (c_int, [c_void_p, c_int_p]),
     "bgpq_threads": (c_int, [c_int]),
@@ -41,6 +41,8 @@ SIGNATURES = {
     "bgpq_default_grid": (c_int, [c_int]),
     "bgpq_test_sort": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_u64, c_int]),
     "bgpq_test_merge": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int]),
+    "bgpq_gen_keys": (c_int, [c_int, c_u64, c_i64, c_void_p, c_void_p]),
+    "bgpq_dev_copy": (c_int, [c_void_p, c_void_p, c_i64, c_void_p]),
     "bgpq_last_error": (ctypes.c_char_p, []),
     "bgpq_dev_malloc": (c_int, [ctypes.POINTER(c_void_p), c_size_t]),
     "bgpq_dev_free": (c_int, [c_void_p]),

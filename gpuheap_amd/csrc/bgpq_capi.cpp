@@ -339,9 +339,10 @@ int bgpq_concurrent(bgpq_heap* h, const void* d_keys, int64_t n_insert, int chun
     return BGPQ_OK;
 }
 
-int bgpq_heapsort(bgpq_heap* h, void* d_keys, int64_t n, int chunk, int grid_insert, int grid_delete,
-                  float* ms_insert, float* ms_delete, void* stream) {
+int bgpq_heapsort(bgpq_heap* h, const void* d_in, void* d_out, int64_t n, int chunk, int grid_insert,
+                  int grid_delete, float* ms_insert, float* ms_delete, void* stream) {
     if (!h) return fail(BGPQ_EINVAL, "null handle");
+    if (n < 0 || (n > 0 && (!d_in || !d_out))) return fail(BGPQ_EINVAL, "bad buffers");
     hipStream_t s = (hipStream_t)stream;
     if (chunk <= 0) chunk = h->batch_size;
     int64_t items = 0;
@@ -358,11 +359,11 @@ int bgpq_heapsort(bgpq_heap* h, void* d_keys, int64_t n, int chunk, int grid_ins
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventCreate(&e2));
     HIPCHK(hipEventRecord(e0, s));
-    rc = bgpq_insert(h, d_keys, n, chunk, grid_insert, stream);
+    rc = bgpq_insert(h, d_in, n, chunk, grid_insert, stream);
     if (!rc) {
         HIPCHK(hipEventRecord(e1, s));
         // enough calls to drain: every call removes a full batch or the partial buffer
-        rc = bgpq_delete(h, d_keys, (n + B - 1) / B + 1, grid_delete, stream);
+        rc = bgpq_delete(h, d_out, (n + B - 1) / B + 1, grid_delete, stream);
     }
     if (!rc) {
         HIPCHK(hipEventRecord(e2, s));

@@ -15,7 +15,7 @@
 // slots 1..batchCount the full sorted batches, every other slot holds
 // init_limits.  One status word per slot and the counters (batchCount,
 // partialBufferSize, deleteCount = heap-sort output cursor, error flags,
-// abort word, insertion ticket) each sit alone in a 128-byte line
+// abort word) each sit alone in a 128-byte line
 // (kWordStride ints apart), so spinning workgroups do not share a line
 // with the words the lock holder updates.
 //
@@ -32,8 +32,7 @@
 namespace bgpq {
 
 enum { VAR_BUITDD = 0, VAR_BUITDD_NP = 1, VAR_TDITDD = 2, VAR_TDITDD_NP = 3 };
-enum { CTR_BATCH_COUNT = 0, CTR_PARTIAL = 1, CTR_DELETE_COUNT = 2, CTR_ERROR = 3, CTR_ABORT = 4, CTR_TICKET = 5,
-       CTR_WORDS = 8 };
+enum { CTR_BATCH_COUNT = 0, CTR_PARTIAL = 1, CTR_DELETE_COUNT = 2, CTR_ERROR = 3, CTR_ABORT = 4, CTR_WORDS = 8 };
 enum { ERR_OVERFLOW = 1, ERR_BAD_SIZE = 2, ERR_LOST_KEYS = 4, ERR_SPIN_LIMIT = 8 };
 
 // Ints between consecutive status words / counters: one 128-byte line each.
@@ -95,7 +94,7 @@ struct DevHeap {
     static constexpr bool PB = (VAR == VAR_BUITDD || VAR == VAR_TDITDD);
     // state words (buitdd:6-9, tditdd:6-9).  bu words carry a tag: an
     // inserter marks the node it is sifting up as (tag << 2) | INSHOLD with a
-    // tag unique to that insertion, and INUSE is INT_MAX so the atomicMax
+    // tag no other live insertion uses, and INUSE is INT_MAX so the atomicMax
     // lock idiom still works.  Finding its own tag again proves that no
     // deleter touched the node in between (see the early stop in insertion).
     static constexpr int AVAIL = 0;
@@ -458,13 +457,12 @@ struct DevHeap {
         if (threadIdx.x == 0) {
             if (lock_cas(0, AVAIL, INUSE)) {
                 bc[BC_A] = a_load(ctr(CTR_BATCH_COUNT));
-                bc[BC_B] = a_load(ctr(CTR_PARTIAL));
-                if constexpr (BU) {
-                    const int ticket = a_load(ctr(CTR_TICKET));
-                    a_store(ctr(CTR_TICKET), ticket + 1);
-                    bc[BC_TAG] = ((ticket & kTagMask) << 2) | INSHOLD;
-                }
+                bc[BC_B] = PB ? a_load(ctr(CTR_PARTIAL)) : 0;
             }
+            // A workgroup runs one insertion at a time and an insertion leaves
+            // no tagged word behind when it ends, so the workgroup index is a
+            // tag no other live insertion carries.
+            if constexpr (BU) bc[BC_TAG] = (((int)blockIdx.x + 1) & kTagMask) << 2 | INSHOLD;
         }
         __syncthreads();
         if (aborted()) return;
@@ -545,7 +543,8 @@ struct DevHeap {
                     bc[BC_C] = -1;
                 } else {
                     a_store(ctr(CTR_BATCH_COUNT), idx);
-                    bc[BC_C] = lock_cas(idx, AVAIL, INUSE) ? idx : -2;
+                    // the new slot is nearly always free: try the CAS first
+                    bc[BC_C] = (a_cas(st(idx), AVAIL, INUSE) || lock_cas(idx, AVAIL, INUSE)) ? idx : -2;
                 }
                 release_st(0, AVAIL);
             }

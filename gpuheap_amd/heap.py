@@ -178,10 +178,12 @@ class Heap:
         check(lib().bgpq_concurrent(self._h, _ptr(d_keys), int(n_insert), int(chunk), _ptr(d_out),
                                     int(calls), int(grid), None))
 
-    def heapsort(self, d_keys, n, chunk=0, grid_insert=0, grid_delete=0):
-        """Sort n device keys in place; returns (ms_insert, ms_delete)."""
+    def heapsort(self, d_keys, n, chunk=0, grid_insert=0, grid_delete=0, d_out=None):
+        """Heap-sort n device keys into d_out (default: in place); returns
+        (ms_insert, ms_delete) measured with HIP events."""
         a, b = ctypes.c_float(), ctypes.c_float()
-        check(lib().bgpq_heapsort(self._h, _ptr(d_keys), int(n), int(chunk), int(grid_insert),
+        out = d_keys if d_out is None else d_out
+        check(lib().bgpq_heapsort(self._h, _ptr(d_keys), _ptr(out), int(n), int(chunk), int(grid_insert),
                                   int(grid_delete), ctypes.byref(a), ctypes.byref(b), None))
         return a.value, b.value
 

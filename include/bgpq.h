@@ -74,10 +74,12 @@ int bgpq_delete(bgpq_heap* h, void* d_out, int64_t calls, int grid, void* stream
  * inserts n_insert keys in chunks, the second half runs `calls` deletes. */
 int bgpq_concurrent(bgpq_heap* h, const void* d_keys, int64_t n_insert, int chunk, void* d_out,
                     int64_t calls, int grid, void* stream);
-/* Heap sort of n device keys in place: insert all (chunk keys per call), then
- * delete until empty.  The heap must be empty.  Optional outputs: the insert
- * and delete phase times in ms (HIP events on `stream`). */
-int bgpq_heapsort(bgpq_heap* h, void* d_keys, int64_t n, int chunk, int grid_insert,
+/* Heap sort: insert the n keys of d_in (chunk keys per insertion() call),
+ * then delete until empty into d_out, which receives the keys ascending
+ * (d_out may equal d_in).  The heap must be empty and hold batch_num >=
+ * 2*ceil(n/batch_size) slots.  Optional outputs: the insert and delete phase
+ * times in ms, from HIP events on `stream`. */
+int bgpq_heapsort(bgpq_heap* h, const void* d_in, void* d_out, int64_t n, int chunk, int grid_insert,
                   int grid_delete, float* ms_insert, float* ms_delete, void* stream);
 
 /* Host copies of the whole device state, for parity tests: items
@@ -102,6 +104,15 @@ int bgpq_test_sort(int key_type, int batch_size, const void* d_in, const int* d_
                    uint64_t pad, int batches);
 int bgpq_test_merge(int key_type, int batch_size, const void* d_a, const void* d_b, void* d_small, void* d_large,
                     int batches, int algo);
+
+/* Synthetic heap-sort keys on the device: key i is built from the (i+1)-th
+ * SplitMix64 output for `seed`; u32 keys lie in [0, 2^32-2], u64 keys are
+ * (u32 key << 32) | i.  Ordered on `stream`. */
+int bgpq_gen_keys(int key_type, uint64_t seed, int64_t n, void* d_out, void* stream);
+
+/* Plain device-to-device copy kernel (bytes a multiple of 16); used to
+ * calibrate HBM byte counters against a kernel of known traffic. */
+int bgpq_dev_copy(void* d_dst, const void* d_src, int64_t bytes, void* stream);
 
 const char* bgpq_last_error(void);
 
