@@ -185,8 +185,10 @@ def main():
     dom_ms = r["ms_" + dom]
     achieved = ab[dom] / (dom_ms / 1e3) / 1e9
     workload = f"heapsort_{args.key}_n{args.n}_{args.variant}_b{args.batch}_{args.order}"
+    measured = traffic_for(workload)
     roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for(workload),
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured.get(f"k_{dom}") if measured else None,
                 "alg_bytes_per_launch": ab[dom]}
     line = {
         "metric": METRIC,
