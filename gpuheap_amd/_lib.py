@@ -31,8 +31,9 @@ SIGNATURES = {
     "bgpq_node_count": (c_int, [c_void_p, c_i64_p]),
     "bgpq_is_empty": (c_int, [c_void_p]),
     "bgpq_insert": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_int, c_void_p]),
-    "bgpq_delete": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_void_p]),
-    "bgpq_concurrent": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_void_p, c_i64, c_int, c_void_p]),
+    "bgpq_delete": (c_int, [c_void_p, c_void_p, c_i64, c_i64, c_int, c_void_p]),
+    "bgpq_set_cursor": (c_int, [c_void_p, c_i64]),
+    "bgpq_concurrent": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_void_p, c_i64, c_i64, c_int, c_void_p]),
     "bgpq_heapsort": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int, c_int, c_float_p, c_float_p, c_void_p]),
     "bgpq_snapshot": (c_int, [c_void_p, c_void_p, c_int_p, c_int_p]),
     "bgpq_error_flags": (c_int, [c_void_p, c_int_p]),

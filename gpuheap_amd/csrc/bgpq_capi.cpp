@@ -80,8 +80,14 @@ static const LaunchRow* rows_for(int key_type, int variant) {
 static size_t status_bytes(int batch_num) { return sizeof(int) * kWordStride * (size_t)(batch_num + 1); }
 static size_t ctr_bytes() { return sizeof(int) * kWordStride * CTR_WORDS; }
 
-static HeapArgs args_of(const bgpq_heap* h) {
-    return HeapArgs{h->items, h->status, h->ctr, h->init_limits, h->batch_num, 1};
+static int env_pack() {
+    const char* v = getenv("BGPQ_XCD_PACK");
+    const int p = v ? atoi(v) : 1;
+    return p >= 1 && p <= 8 ? p : 1;
+}
+
+static HeapArgs args_of(const bgpq_heap* h, long long out_cap = 0) {
+    return HeapArgs{h->items, h->status, h->ctr, h->init_limits, h->batch_num, 1, out_cap, env_pack()};
 }
 
 static int fill_items(bgpq_heap* h, hipStream_t s) {
@@ -227,6 +233,13 @@ int bgpq_reset(bgpq_heap* h, void* stream) {
     return fill_items(h, (hipStream_t)stream);
 }
 
+int bgpq_set_cursor(bgpq_heap* h, int64_t value) {
+    if (!h || value < 0 || value > 0x7FFFFFFF) return fail(BGPQ_EINVAL, "bad cursor");
+    const int v = (int)value;
+    HIPCHK(hipMemcpy(h->ctr + CTR_DELETE_COUNT * kWordStride, &v, sizeof v, hipMemcpyHostToDevice));
+    return BGPQ_OK;
+}
+
 int bgpq_error_flags(bgpq_heap* h, int* out) {
     if (!h || !out) return fail(BGPQ_EINVAL, "null argument");
     int c[CTR_WORDS];
@@ -313,18 +326,19 @@ int bgpq_insert(bgpq_heap* h, const void* d_keys, int64_t n, int chunk, int grid
     return BGPQ_OK;
 }
 
-int bgpq_delete(bgpq_heap* h, void* d_out, int64_t calls, int grid, void* stream) {
+int bgpq_delete(bgpq_heap* h, void* d_out, int64_t out_keys, int64_t calls, int grid, void* stream) {
     if (!h) return fail(BGPQ_EINVAL, "null handle");
-    if (calls < 0) return fail(BGPQ_EINVAL, "calls %lld", (long long)calls);
+    if (calls < 0 || out_keys < 0) return fail(BGPQ_EINVAL, "calls %lld, out_keys %lld", (long long)calls,
+                                               (long long)out_keys);
     if (calls == 0) return BGPQ_OK;
     if (!d_out) return fail(BGPQ_EINVAL, "null output");
-    hipError_t e = h->row->del(args_of(h), d_out, calls, resolve_grid(h, grid), (hipStream_t)stream);
+    hipError_t e = h->row->del(args_of(h, out_keys), d_out, calls, resolve_grid(h, grid), (hipStream_t)stream);
     if (e != hipSuccess) return fail(BGPQ_EHIP, "delete launch: %s", hipGetErrorString(e));
     return BGPQ_OK;
 }
 
-int bgpq_concurrent(bgpq_heap* h, const void* d_keys, int64_t n_insert, int chunk, void* d_out, int64_t calls,
-                    int grid, void* stream) {
+int bgpq_concurrent(bgpq_heap* h, const void* d_keys, int64_t n_insert, int chunk, void* d_out, int64_t out_keys,
+                    int64_t calls, int grid, void* stream) {
     if (!h) return fail(BGPQ_EINVAL, "null handle");
     if (n_insert < 0 || calls < 0) return fail(BGPQ_EINVAL, "negative size");
     if (chunk <= 0) chunk = h->batch_size;
@@ -334,7 +348,8 @@ int bgpq_concurrent(bgpq_heap* h, const void* d_keys, int64_t n_insert, int chun
     grid = resolve_grid(h, grid);
     if (grid < 2) return fail(BGPQ_EINVAL, "concurrent run needs a grid of at least 2");
     if ((n_insert && !d_keys) || (calls && !d_out)) return fail(BGPQ_EINVAL, "null buffer");
-    hipError_t e = h->row->conc(args_of(h), d_keys, n_insert, chunk, d_out, calls, grid, (hipStream_t)stream);
+    if (out_keys < 0) return fail(BGPQ_EINVAL, "out_keys %lld", (long long)out_keys);
+    hipError_t e = h->row->conc(args_of(h, out_keys), d_keys, n_insert, chunk, d_out, calls, grid, (hipStream_t)stream);
     if (e != hipSuccess) return fail(BGPQ_EHIP, "concurrent launch: %s", hipGetErrorString(e));
     return BGPQ_OK;
 }
@@ -343,6 +358,7 @@ int bgpq_heapsort(bgpq_heap* h, const void* d_in, void* d_out, int64_t n, int ch
                   int grid_delete, float* ms_insert, float* ms_delete, void* stream) {
     if (!h) return fail(BGPQ_EINVAL, "null handle");
     if (n < 0 || (n > 0 && (!d_in || !d_out))) return fail(BGPQ_EINVAL, "bad buffers");
+    if (n > 0x7FFFFFFF) return fail(BGPQ_EINVAL, "the heap-sort cursor is 32-bit: n <= 2^31-1");
     hipStream_t s = (hipStream_t)stream;
     if (chunk <= 0) chunk = h->batch_size;
     int64_t items = 0;
@@ -363,7 +379,7 @@ int bgpq_heapsort(bgpq_heap* h, const void* d_in, void* d_out, int64_t n, int ch
     if (!rc) {
         HIPCHK(hipEventRecord(e1, s));
         // enough calls to drain: every call removes a full batch or the partial buffer
-        rc = bgpq_delete(h, d_out, (n + B - 1) / B + 1, grid_delete, stream);
+        rc = bgpq_delete(h, d_out, n, (n + B - 1) / B + 1, grid_delete, stream);
     }
     if (!rc) {
         HIPCHK(hipEventRecord(e2, s));

@@ -29,11 +29,17 @@
 #pragma once
 #include "bgpq_prims.hpp"
 
+// Phase hook for timeline tracing builds: such a build defines
+// BGPQ_EVENT(id) before including this header.  By default it is empty.
+#ifndef BGPQ_EVENT
+#define BGPQ_EVENT(id) ((void)0)
+#endif
+
 namespace bgpq {
 
 enum { VAR_BUITDD = 0, VAR_BUITDD_NP = 1, VAR_TDITDD = 2, VAR_TDITDD_NP = 3 };
 enum { CTR_BATCH_COUNT = 0, CTR_PARTIAL = 1, CTR_DELETE_COUNT = 2, CTR_ERROR = 3, CTR_ABORT = 4, CTR_WORDS = 8 };
-enum { ERR_OVERFLOW = 1, ERR_BAD_SIZE = 2, ERR_LOST_KEYS = 4, ERR_SPIN_LIMIT = 8 };
+enum { ERR_OVERFLOW = 1, ERR_BAD_SIZE = 2, ERR_LOST_KEYS = 4, ERR_SPIN_LIMIT = 8, ERR_OUTPUT_FULL = 16 };
 
 // Ints between consecutive status words / counters: one 128-byte line each.
 constexpr int kWordStride = 32;
@@ -51,7 +57,8 @@ struct HeapView {
     int* ctr;     // CTR_WORDS * kWordStride ints
     K init_limits;
     int batch_num;
-    int heap_sort;  // nonzero: deleteRoot writes at items + deleteCount (HEAP_SORT mode)
+    int heap_sort;     // nonzero: deleteRoot writes at items + deleteCount (HEAP_SORT mode)
+    long long out_cap; // heap-sort mode: keys the caller's output buffer holds
 };
 
 // Slot <-> logical node map of the bu variants.  rev(1) is defined as 1: the
@@ -105,6 +112,9 @@ struct DevHeap {
     __device__ __forceinline__ static bool is_inshold(int w) { return w != INUSE && (w & 3) == INSHOLD; }
 
     HeapView<K> h;
+    // Set by deleteRoot when it already took the last batch into LDS (and
+    // folded the partial buffer in): the slot it took, 0 otherwise.
+    mutable int pre_last_ = 0;
 
     __device__ __forceinline__ K* node(int i) const { return h.items + (size_t)i * B; }
     __device__ __forceinline__ int* st(int i) const { return h.status + (size_t)i * kWordStride; }
@@ -130,7 +140,10 @@ struct DevHeap {
         __hip_atomic_fetch_or(ctr(CTR_ERROR), (int)ERR_SPIN_LIMIT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         bcast_slots()[BC_ABORT] = 1;
     }
+    // The first attempt goes straight to the atomic (an uncontended lock costs
+    // one round trip); only a failed attempt falls back to polling.
     __device__ __forceinline__ bool lock_cas(int i, int from, int to) const {
+        if (a_cas(st(i), from, to)) return true;
         int n = 0;
         while (!(a_load(st(i)) == from && a_cas(st(i), from, to))) {
             if (poll_abort(n)) return false;
@@ -141,10 +154,12 @@ struct DevHeap {
     // bu "lock whatever state, remember it" idiom (buitdd:372-377,409-418);
     // returns the previous state, or -1 on abort
     __device__ __forceinline__ int lock_max(int i) const {
+        int p = a_max(st(i), INUSE);
+        if (p != INUSE) return p;
         int n = 0;
         while (true) {
             if (a_load(st(i)) != INUSE) {
-                const int p = a_max(st(i), INUSE);
+                p = a_max(st(i), INUSE);
                 if (p != INUSE) return p;
             }
             if (poll_abort(n)) return -1;
@@ -154,9 +169,11 @@ struct DevHeap {
     // td child lock: AVAIL -> INUSE, waiting while INUSE; returns the state
     // found (AVAIL means locked), or -1 on abort
     __device__ __forceinline__ int lock_child_td(int i) const {
+        int s = a_cas_val(st(i), AVAIL, INUSE);
+        if (s != INUSE) return s;
         int n = 0;
         while (true) {
-            const int s = a_load(st(i));
+            s = a_load(st(i));
             if (s != INUSE) {
                 if (s != AVAIL) return s;
                 if (a_cas(st(i), AVAIL, INUSE)) return AVAIL;
@@ -207,13 +224,24 @@ struct DevHeap {
     // Lock the partial buffer (status[0]); if the heap holds full batches,
     // also lock the root and hand the root batch to the caller.  Returns true
     // when deleteUpdate must follow (both locks stay held until then).
-    __device__ bool deleteRoot(K* items, int& size) const {
+    //
+    // With smemOffset >= 0 (bu variants), the deleter also takes the last
+    // batch as its candidate, and folds the partial buffer into it, *before*
+    // it waits for the root: that work then overlaps the previous deleter's
+    // sift instead of following it.  status[0] is still held until the root
+    // is locked, so roots are taken in heap-sort cursor order.  deleteUpdate
+    // must then be called with the same smemOffset.
+    __device__ bool deleteRoot(K* items, int& size, int smemOffset = -1) const {
         int* bc = bcast_slots();
         size = 0;
+        pre_last_ = 0;
         if (!op_begin()) return false;
+        const bool prefetch = BU && smemOffset >= 0;
+        BGPQ_EVENT(1);
         if (threadIdx.x == 0) {
             do {
                 if (!lock_cas(0, AVAIL, INUSE)) break;
+                BGPQ_EVENT(2);
                 const int nb = a_load(ctr(CTR_BATCH_COUNT));
                 const int pbs = a_load(ctr(CTR_PARTIAL));
                 const int off = h.heap_sort ? a_load(ctr(CTR_DELETE_COUNT)) : 0;
@@ -221,18 +249,37 @@ struct DevHeap {
                 if (nb == 0 && pbs == 0) mode = 0;
                 else if (nb == 0) mode = PB ? 1 : 0;
                 else mode = 2;
-                if (mode == 2) {
+                // the heap-sort cursor never writes past the caller's buffer:
+                // refuse the deletion and flag it instead
+                const long long need = mode == 2 ? B : (mode == 1 ? pbs : 0);
+                if (h.heap_sort && mode != 0 && (long long)off + need > h.out_cap) {
+                    __hip_atomic_fetch_or(ctr(CTR_ERROR), (int)ERR_OUTPUT_FULL, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                    mode = 0;
+                }
+                // Only a batch below the root's children may leave early: the
+                // previous deleter may still be refilling the root from slots
+                // 2 and 3, while any deeper batch only holds keys >= its
+                // level-1 ancestor, hence >= everything the new root receives.
+                int last = 0;
+                if (mode == 2 && prefetch && nb >= 4) {
+                    last = nb;
+                    a_store(ctr(CTR_BATCH_COUNT), nb - 1);  // every change of it happens under status[0]
+                    if (lock_max(last) < 0) break;
+                    BGPQ_EVENT(3);
+                } else if (mode == 2) {
                     if (!lock_cas(1, AVAIL, INUSE)) break;
                     if (h.heap_sort) a_store(ctr(CTR_DELETE_COUNT), off + B);
                 }
                 bc[BC_A] = mode;
                 bc[BC_B] = pbs;
                 bc[BC_C] = off;
+                bc[BC_D] = last;
             } while (0);
         }
         __syncthreads();
         if (aborted()) return false;
-        const int mode = bc[BC_A], pbs = bc[BC_B], off = bc[BC_C];
+        const int mode = bc[BC_A], pbs = bc[BC_B], off = bc[BC_C], last = bc[BC_D];
         __syncthreads();
         if (mode == 0) {
             if (threadIdx.x == 0) release_st(0, AVAIL);
@@ -254,6 +301,31 @@ struct DevHeap {
             size = pbs;
             return false;
         }
+        if (last > 0) {
+            // candidate = the last batch (+ the partial buffer's smaller half),
+            // taken while the previous deleter still holds the root
+            K* X0 = lds_ws(smemOffset);
+            K* X1 = X0 + B;
+            take_batch<K, B, NT>(X0, node(last), h.init_limits);
+            publish();
+            BGPQ_EVENT(4);
+            if (threadIdx.x == 0) release_st(last, AVAIL);
+            if constexpr (PB) {
+                g2l_batch<K, B, NT>(X1, node(0));
+                __syncthreads();
+                wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
+                publish();
+            }
+            if (threadIdx.x == 0) {
+                lock_cas(1, AVAIL, INUSE);
+                BGPQ_EVENT(5);
+                if (h.heap_sort) a_store(ctr(CTR_DELETE_COUNT), off + B);
+                release_st(0, AVAIL);
+            }
+            __syncthreads();
+            if (aborted()) return false;
+            pre_last_ = last;
+        }
         K* dst = items + off;
         if (((uintptr_t)dst % 16) == 0) {
             g2g_batch<K, B, NT>(dst, node(1));
@@ -262,6 +334,7 @@ struct DevHeap {
         }
         size = B;
         __syncthreads();
+        BGPQ_EVENT(6);
         return true;
     }
 
@@ -272,6 +345,12 @@ struct DevHeap {
         K* X2 = X1 + B;
         int* bc = bcast_slots();
         if (!op_begin()) return;
+        if (pre_last_ > 0) {
+            // deleteRoot already took the candidate and released status[0]
+            pre_last_ = 0;
+            sift_down(X0, X1, X2);
+            return;
+        }
 
         if (threadIdx.x == 0) {
             const int last = a_add(ctr(CTR_BATCH_COUNT), -1);
@@ -332,13 +411,18 @@ struct DevHeap {
             __syncthreads();
         }
         if (threadIdx.x == 0) release_st(0, AVAIL);
+        sift_down(X0, X1, X2);
+    }
 
-        // ---- sift the candidate down from the root
+    // ---- sift the candidate X0 down from the root, which the caller holds
+    __device__ void sift_down(K* X0, K* X1, K* X2) const {
+        int* bc = bcast_slots();
         int cur = 1;
         int cur_prev = AVAIL;
         while (true) {
             const int l = left_child(cur), r = right_child(cur);
             const bool lin = l <= h.batch_num, rin = r <= h.batch_num;
+            BGPQ_EVENT(10);
             if constexpr (BU) {
                 // lanes 0 and 1 lock the two children at the same time
                 if (threadIdx.x < 2) {
@@ -368,6 +452,7 @@ struct DevHeap {
                 }
             }
             __syncthreads();
+            BGPQ_EVENT(11);
             if (aborted()) return;
             const int lp = bc[BC_C], rp = bc[BC_D], type = bc[BC_E];
             if (type != 0) {
@@ -379,6 +464,7 @@ struct DevHeap {
                 else fill_lds<K, B, NT>(X2, h.init_limits);
             }
             __syncthreads();
+            BGPQ_EVENT(12);
 
             if constexpr (!BU) {
                 if (type == 0) {  // no child: the candidate stays here
@@ -407,6 +493,7 @@ struct DevHeap {
             const int tgt_prev = tr ? rp : lp, oth_prev = tr ? lp : rp;
             if (tgt_in) wg_merge<K, B, NT>(X1, X2, ToLds<K>{X1}, ToGlobal<K>{node(tgt)});
             else wg_merge<K, B, NT>(X1, X2, ToLds<K>{X1}, ToNone{});
+            BGPQ_EVENT(13);
 
             const K cmin = X0[0], cmax = X0[B - 1], kmin_ = X1[0], kmax_ = X1[B - 1];
             const bool stop = (VAR == VAR_BUITDD) ? (cmax < kmin_) : !(kmin_ < cmax);
@@ -422,15 +509,18 @@ struct DevHeap {
                     release_st(cur, BU ? cur_prev : AVAIL);
                     if (oth_in) release_st(oth, BU ? oth_prev : AVAIL);
                 }
+                BGPQ_EVENT(16);
                 return;
             } else {
                 wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(cur)}, ToLds<K>{X0});
             }
+            BGPQ_EVENT(14);
             publish();
             if (threadIdx.x == 0) {
                 if (tgt_in) release_st(tgt, BU ? tgt_prev : AVAIL);
                 release_st(cur, BU ? cur_prev : AVAIL);
             }
+            BGPQ_EVENT(15);
             if (!oth_in) {
                 // only reachable when keys equal init_limits (outside the contract)
                 flag_error(ERR_LOST_KEYS);
@@ -452,10 +542,13 @@ struct DevHeap {
             return;
         }
         if (!op_begin()) return;
+        BGPQ_EVENT(20);
         wg_sort_batch<K, B, NT>(X0, items, size, h.init_limits);
+        BGPQ_EVENT(21);
 
         if (threadIdx.x == 0) {
             if (lock_cas(0, AVAIL, INUSE)) {
+                BGPQ_EVENT(22);
                 bc[BC_A] = a_load(ctr(CTR_BATCH_COUNT));
                 bc[BC_B] = PB ? a_load(ctr(CTR_PARTIAL)) : 0;
             }
@@ -556,6 +649,7 @@ struct DevHeap {
                 flag_error(ERR_OVERFLOW);
                 return;
             }
+            BGPQ_EVENT(23);
             l2g_batch<K, B, NT>(node(cur), X0);
             publish();
             if (threadIdx.x == 0) release_st(cur, cur == 1 ? AVAIL : mine);
@@ -571,15 +665,18 @@ struct DevHeap {
                 const int par = parent(cur);
                 if (threadIdx.x == 0) {
                     int prev = -1;
+                    BGPQ_EVENT(24);
                     if (lock_cas(par, AVAIL, INUSE)) prev = lock_max(cur);
                     bc[BC_D] = prev;
                 }
                 __syncthreads();
+                BGPQ_EVENT(25);
                 if (aborted()) return;
                 dirty = dirty || (bc[BC_D] != mine);
                 g2l_batch<K, B, NT>(X0, node(cur));
                 g2l_batch<K, B, NT>(X1, node(par));
                 __syncthreads();
+                BGPQ_EVENT(26);
                 const bool identity = !(X0[0] < X1[B - 1]);  // parent.max <= node.min
                 const bool done = identity && !dirty;
                 if (!identity) {
@@ -596,6 +693,7 @@ struct DevHeap {
                     release_st(par, (done || par == 1) ? AVAIL : mine);
                     release_st(cur, AVAIL);
                 }
+                BGPQ_EVENT(done ? 28 : 27);
                 if (done) return;
                 cur = par;
             }

@@ -37,6 +37,7 @@ hipError_t fill_u64(void* p, long long n, uint64_t v, hipStream_t s) {
     long long blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) blocks = 1;
+    (void)hipGetLastError();  // report this launch only
     hipLaunchKernelGGL(k_fill<KeyT>, dim3((unsigned)blocks), dim3(256), 0, s, (KeyT*)p, n, (KeyT)v);
     return hipGetLastError();
 }

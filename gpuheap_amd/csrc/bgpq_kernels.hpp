@@ -12,53 +12,72 @@ namespace bgpq {
 // Workgroup size per batch size.  Every thread holds B/NT keys while sorting
 // and produces 2B/NT keys per merge.
 constexpr int threads_for(int B) {
-    return B <= 128 ? 64 : (B == 256 ? 128 : (B <= 1024 ? 256 : 512));
+    return B <= 128 ? 64 : B / 2;
 }
+
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
+// XCD b mod 8).  With pack > 1 only every pack-th workgroup of the launch
+// works, so pack = 8 keeps every worker on one XCD, whose L2 then serves the
+// coherent accesses the workers make to each other's nodes.
+struct Workers {
+    int id, count;
+    __device__ __forceinline__ static bool of(int pack, Workers& w) {
+        if ((int)blockIdx.x % pack != 0) return false;
+        w.id = (int)blockIdx.x / pack;
+        w.count = (int)gridDim.x / pack;
+        return true;
+    }
+};
 
 template <typename K, int VAR, int B>
 __global__ __launch_bounds__(threads_for(B)) void k_insert(HeapView<K> h, const K* keys, long long n,
-                                                           int chunk) {
+                                                           int chunk, int pack) {
     constexpr int NT = threads_for(B);
+    Workers w;
+    if (!Workers::of(pack, w)) return;
     DevHeap<K, VAR, B, NT> hp{h};
     const long long calls = (n + chunk - 1) / chunk;
-    for (long long i = blockIdx.x; i < calls; i += gridDim.x) {
+    for (long long i = w.id; i < calls; i += w.count) {
         const long long rem = n - i * chunk;
         hp.insertion(keys + i * chunk, (int)(rem < chunk ? rem : chunk), 0);
     }
 }
 
 template <typename K, int VAR, int B>
-__global__ __launch_bounds__(threads_for(B)) void k_delete(HeapView<K> h, K* out, long long calls) {
+__global__ __launch_bounds__(threads_for(B)) void k_delete(HeapView<K> h, K* out, long long calls, int pack) {
     constexpr int NT = threads_for(B);
+    Workers w;
+    if (!Workers::of(pack, w)) return;
     DevHeap<K, VAR, B, NT> hp{h};
-    for (long long i = blockIdx.x; i < calls; i += gridDim.x) {
+    for (long long i = w.id; i < calls; i += w.count) {
         int size = 0;
-        if (hp.deleteRoot(out, size)) hp.deleteUpdate(0);
+        if (hp.deleteRoot(out, size, 0)) hp.deleteUpdate(0);
     }
 }
 
 template <typename K, int VAR, int B>
 __global__ __launch_bounds__(threads_for(B)) void k_concurrent(HeapView<K> h, const K* keys, long long n,
-                                                               int chunk, K* out, long long calls) {
+                                                               int chunk, K* out, long long calls, int pack) {
     constexpr int NT = threads_for(B);
+    Workers w;
+    if (!Workers::of(pack, w)) return;
     DevHeap<K, VAR, B, NT> hp{h};
-    const int half = gridDim.x / 2;
-    if ((int)blockIdx.x < half) {
+    const int half = w.count / 2;
+    if (w.id < half) {
         const long long ins = (n + chunk - 1) / chunk;
-        for (long long i = blockIdx.x; i < ins; i += half) {
+        for (long long i = w.id; i < ins; i += half) {
             const long long rem = n - i * chunk;
             hp.insertion(keys + i * chunk, (int)(rem < chunk ? rem : chunk), 0);
         }
     } else {
-        const int nd = gridDim.x - half;
-        for (long long i = blockIdx.x - half; i < calls; i += nd) {
+        const int nd = w.count - half;
+        for (long long i = w.id - half; i < calls; i += nd) {
             int size = 0;
-            if (hp.deleteRoot(out, size)) hp.deleteUpdate(0);
+            if (hp.deleteRoot(out, size, 0)) hp.deleteUpdate(0);
         }
     }
 }
 
-// Primitive test kernels: one workgroup per batch.
 template <typename K, int B>
 __global__ __launch_bounds__(threads_for(B)) void k_prim_sort(const K* in, const int* sizes, K* out, K pad) {
     constexpr int NT = threads_for(B);
@@ -98,6 +117,8 @@ struct HeapArgs {
     uint64_t init_limits;
     int batch_num;
     int heap_sort;
+    long long out_cap;
+    int pack;  // workgroups per worker (Workers); 1 = every workgroup works
 };
 
 struct LaunchRow {
@@ -112,8 +133,12 @@ struct LaunchRow {
 
 template <typename K>
 inline HeapView<K> view_of(const HeapArgs& a) {
-    return HeapView<K>{(K*)a.items, a.status, a.ctr, (K)a.init_limits, a.batch_num, a.heap_sort};
+    return HeapView<K>{(K*)a.items, a.status, a.ctr, (K)a.init_limits, a.batch_num, a.heap_sort, a.out_cap};
 }
+
+// hipGetLastError reports the most recent failure of any runtime call on the
+// thread; it is cleared before each launch so that a launch reports its own.
+inline void clear_last_error() { (void)hipGetLastError(); }
 
 template <typename K, int VAR, int B>
 struct Launch {
@@ -121,28 +146,33 @@ struct Launch {
     static_assert(delete_lds_bytes<K, B>() <= 64 * 1024, "dynamic LDS per workgroup");
     static hipError_t insert(const HeapArgs& a, const void* keys, long long n, int chunk, int grid,
                              hipStream_t s) {
-        hipLaunchKernelGGL((k_insert<K, VAR, B>), dim3(grid), dim3(threads_for(B)), lds, s, view_of<K>(a),
-                           (const K*)keys, n, chunk);
+        clear_last_error();
+        hipLaunchKernelGGL((k_insert<K, VAR, B>), dim3(grid * a.pack), dim3(threads_for(B)), lds, s, view_of<K>(a),
+                           (const K*)keys, n, chunk, a.pack);
         return hipGetLastError();
     }
     static hipError_t del(const HeapArgs& a, void* out, long long calls, int grid, hipStream_t s) {
-        hipLaunchKernelGGL((k_delete<K, VAR, B>), dim3(grid), dim3(threads_for(B)), lds, s, view_of<K>(a),
-                           (K*)out, calls);
+        clear_last_error();
+        hipLaunchKernelGGL((k_delete<K, VAR, B>), dim3(grid * a.pack), dim3(threads_for(B)), lds, s, view_of<K>(a),
+                           (K*)out, calls, a.pack);
         return hipGetLastError();
     }
     static hipError_t conc(const HeapArgs& a, const void* keys, long long n, int chunk, void* out,
                            long long calls, int grid, hipStream_t s) {
-        hipLaunchKernelGGL((k_concurrent<K, VAR, B>), dim3(grid), dim3(threads_for(B)), lds, s, view_of<K>(a),
-                           (const K*)keys, n, chunk, (K*)out, calls);
+        clear_last_error();
+        hipLaunchKernelGGL((k_concurrent<K, VAR, B>), dim3(grid * a.pack), dim3(threads_for(B)), lds, s,
+                           view_of<K>(a), (const K*)keys, n, chunk, (K*)out, calls, a.pack);
         return hipGetLastError();
     }
     static hipError_t sort(const void* in, const int* sizes, void* out, uint64_t pad, int batches, hipStream_t s) {
+        clear_last_error();
         hipLaunchKernelGGL((k_prim_sort<K, B>), dim3(batches), dim3(threads_for(B)), B * sizeof(K), s, (const K*)in,
                            sizes, (K*)out, (K)pad);
         return hipGetLastError();
     }
     static hipError_t merge(const void* a, const void* b, void* small, void* large, int batches, int algo,
                             hipStream_t s) {
+        clear_last_error();
         if (algo == 0)
             hipLaunchKernelGGL((k_prim_merge<K, B, 0>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
                                (const K*)a, (const K*)b, (K*)small, (K*)large);

@@ -40,6 +40,7 @@ extern "C" int bgpq_gen_keys(int key_type, uint64_t seed, int64_t n, void* d_out
     if (n == 0) return BGPQ_OK;
     long long blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
+    (void)hipGetLastError();  // report this launch only
     if (key_type == BGPQ_KEY_U32)
         hipLaunchKernelGGL(k_gen_u32, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, seed, (long long)n,
                            (uint32_t*)d_out);
@@ -61,6 +62,7 @@ __global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst,
 extern "C" int bgpq_dev_copy(void* d_dst, const void* d_src, int64_t bytes, void* stream) {
     if (bytes < 0 || bytes % 16 || (bytes && (!d_dst || !d_src))) return BGPQ_EINVAL;
     if (!bytes) return BGPQ_OK;
+    (void)hipGetLastError();  // report this launch only
     hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, (hipStream_t)stream, (const uint4*)d_src, (uint4*)d_dst,
                        (long long)(bytes / 16));
     return hipGetLastError() == hipSuccess ? BGPQ_OK : BGPQ_EHIP;

@@ -171,12 +171,24 @@ class Heap:
     def insert(self, d_keys, n, chunk=0, grid=0):
         check(lib().bgpq_insert(self._h, _ptr(d_keys), int(n), int(chunk), int(grid), None))
 
-    def delete(self, d_out, calls, grid=0):
-        check(lib().bgpq_delete(self._h, _ptr(d_out), int(calls), int(grid), None))
+    def delete(self, d_out, calls, grid=0, out_keys=None):
+        """deleteKernel: `calls` deletions writing at the heap-sort cursor;
+        out_keys defaults to the capacity of d_out."""
+        check(lib().bgpq_delete(self._h, _ptr(d_out), self._cap(d_out, out_keys), int(calls), int(grid), None))
 
-    def concurrent(self, d_keys, n_insert, d_out, calls, chunk=0, grid=0):
+    def concurrent(self, d_keys, n_insert, d_out, calls, chunk=0, grid=0, out_keys=None):
         check(lib().bgpq_concurrent(self._h, _ptr(d_keys), int(n_insert), int(chunk), _ptr(d_out),
-                                    int(calls), int(grid), None))
+                                    self._cap(d_out, out_keys), int(calls), int(grid), None))
+
+    def set_cursor(self, value=0):
+        check(lib().bgpq_set_cursor(self._h, int(value)))
+
+    def _cap(self, d_out, out_keys):
+        if out_keys is not None:
+            return int(out_keys)
+        if isinstance(d_out, DeviceBuffer):
+            return d_out.nbytes // self.dtype.itemsize
+        raise ValueError("out_keys is required for a raw device pointer")
 
     def heapsort(self, d_keys, n, chunk=0, grid_insert=0, grid_delete=0, d_out=None):
         """Heap-sort n device keys into d_out (default: in place); returns

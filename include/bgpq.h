@@ -67,13 +67,17 @@ int bgpq_is_empty(bgpq_heap* h);
 int bgpq_insert(bgpq_heap* h, const void* d_keys, int64_t n, int chunk, int grid, void* stream);
 /* Bulk deletion: the harness deleteKernel (test/heap_test.cu:57-74).  Runs
  * `calls` deleteRoot(+deleteUpdate) calls over `grid` workgroups; deleted
- * keys land in d_out at the heap-sort cursor, so d_out receives the keys in
- * ascending order. */
-int bgpq_delete(bgpq_heap* h, void* d_out, int64_t calls, int grid, void* stream);
+ * keys land in d_out at the heap-sort cursor (deleteCount), so d_out
+ * receives the keys in ascending order.  The cursor persists across calls
+ * (bgpq_set_cursor moves it); d_out holds out_keys keys, and a deletion that
+ * would write past them is refused and flags error 16 instead. */
+int bgpq_delete(bgpq_heap* h, void* d_out, int64_t out_keys, int64_t calls, int grid, void* stream);
+/* Set the heap-sort output cursor (deleteCount). */
+int bgpq_set_cursor(bgpq_heap* h, int64_t value);
 /* concurrentKernel (test/heap_test.cu:13-38): the first half of the grid
  * inserts n_insert keys in chunks, the second half runs `calls` deletes. */
 int bgpq_concurrent(bgpq_heap* h, const void* d_keys, int64_t n_insert, int chunk, void* d_out,
-                    int64_t calls, int grid, void* stream);
+                    int64_t out_keys, int64_t calls, int grid, void* stream);
 /* Heap sort: insert the n keys of d_in (chunk keys per insertion() call),
  * then delete until empty into d_out, which receives the keys ascending
  * (d_out may equal d_in).  The heap must be empty and hold batch_num >=
@@ -86,7 +90,8 @@ int bgpq_heapsort(bgpq_heap* h, const void* d_in, void* d_out, int64_t n, int ch
  * ((batch_num+1)*batch_size keys), status (batch_num+1 ints), counters (8 ints:
  * batchCount, partialBufferSize, deleteCount, error flags).  Any may be NULL. */
 int bgpq_snapshot(bgpq_heap* h, void* h_items, int* h_status, int* h_counters);
-/* Device error flags (1 overflow, 2 bad insert size, 4 keys lost). */
+/* Device error flags (1 overflow, 2 bad insert size, 4 keys lost, 8 spin limit,
+ * 16 output buffer full). */
 int bgpq_error_flags(bgpq_heap* h, int* out);
 
 /* Parameters of the kernels built for a batch size. */

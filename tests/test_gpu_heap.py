@@ -184,3 +184,27 @@ def test_empty_delete_and_errors(gpu):
     big = Heap(4, 256, variant="buitdd")
     with pytest.raises(G.BgpqError):
         big.heapsort(DeviceBuffer(4 * 256 * 4), 4 * 256)  # needs batch_num >= 2 * batches
+
+
+def test_output_capacity_guard(gpu):
+    """The heap-sort cursor never writes past the caller's output buffer: a
+    deletion that would not fit is refused, flags error 16 and leaves its keys
+    in the heap."""
+    B = 256
+    n = 8 * B
+    keys = O.keys_u32(77, n)
+    h = Heap(64, B, variant="buitdd_np")
+    h.insert(DeviceBuffer.from_numpy(keys), n)
+    small = DeviceBuffer(3 * B * 4)
+    h.delete(small, 8, grid=4)
+    G.sync()
+    assert h.error_flags() & 16
+    assert h.itemCount() == n - 3 * B
+    srt = np.sort(keys)
+    assert np.array_equal(small.to_numpy(np.uint32, 3 * B), srt[:3 * B])
+    h.set_cursor(0)
+    rest = DeviceBuffer((n - 3 * B) * 4)
+    h.delete(rest, 8, grid=4)
+    G.sync()
+    assert h.isEmpty()
+    assert np.array_equal(rest.to_numpy(np.uint32, n - 3 * B), srt[3 * B:])
