@@ -146,7 +146,7 @@ def main():
     ap.add_argument("--n", type=parse_n, default=1 << 28, help="keys (or log2 of keys if < 64)")
     ap.add_argument("--key", choices=sorted(KEYS), default="u32")
     ap.add_argument("--variant", choices=sorted(G.VARIANTS), default="buitdd_np")
-    ap.add_argument("--batch", type=int, default=2048)
+    ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--grid-insert", type=int, default=256)
     ap.add_argument("--grid-delete", type=int, default=32)
     ap.add_argument("--order", choices=["random", "ascending", "descending"], default="random")
@@ -165,7 +165,7 @@ def main():
     if args.sweep:
         for key, order in (("u32", "random"), ("u32", "descending"), ("u64", "random")):
             for variant in ("buitdd", "buitdd_np", "tditdd", "tditdd_np"):
-                for b in (256, 512, 1024, 2048):
+                for b in (256, 512, 1024, 2048, 4096, 8192):
                     r = run_config(args.n, key, variant, b, args.grid_insert, args.grid_delete, 1, 0, order=order)
                     ab = alg_bytes(args.n, b, KEYS[key][2])
                     r.update({"n": args.n, "key": key, "order": order, "variant": variant, "batch": b,

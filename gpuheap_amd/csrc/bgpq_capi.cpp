@@ -167,6 +167,7 @@ int bgpq_threads(int batch_size) {
 
 int bgpq_lds_bytes(int key_type, int batch_size) {
     if (batch_index(batch_size) < 0) return fail(BGPQ_EINVAL, "batch size %d not built", batch_size);
+    if (key_type != BGPQ_KEY_U32 && key_type != BGPQ_KEY_U64) return fail(BGPQ_EINVAL, "key type %d", key_type);
     return 3 * batch_size * (int)key_bytes(key_type);
 }
 
@@ -186,7 +187,10 @@ int bgpq_create(bgpq_heap** out, int variant, int key_type, int batch_num, int b
     if (variant < 0 || variant > 3) return fail(BGPQ_EINVAL, "variant %d", variant);
     if (key_type != BGPQ_KEY_U32 && key_type != BGPQ_KEY_U64) return fail(BGPQ_EINVAL, "key type %d", key_type);
     const int bi = batch_index(batch_size);
-    if (bi < 0) return fail(BGPQ_EINVAL, "batch size %d (need a power of two in [64, 2048])", batch_size);
+    if (bi < 0) return fail(BGPQ_EINVAL, "batch size %d (need a power of two in [64, 8192])", batch_size);
+    if (rows_for(key_type, variant)[bi].batch_size == 0)
+        return fail(BGPQ_EINVAL, "batch size %d: %d bytes of LDS per workgroup exceed %d", batch_size,
+                    3 * batch_size * (int)key_bytes(key_type), kMaxLdsBytes);
     if (batch_num < 1) return fail(BGPQ_EINVAL, "batch_num %d", batch_num);
     if (key_type == BGPQ_KEY_U32 && init_limits > 0xFFFFFFFFull)
         return fail(BGPQ_EINVAL, "init_limits does not fit u32");
@@ -401,7 +405,8 @@ int bgpq_test_sort(int key_type, int batch_size, const void* d_in, const int* d_
                    uint64_t pad, int batches) {
     const int bi = batch_index(batch_size);
     const LaunchRow* rows = rows_for(key_type, 0);
-    if (bi < 0 || !rows || batches < 1) return fail(BGPQ_EINVAL, "bad primitive test arguments");
+    if (bi < 0 || !rows || batches < 1 || rows[bi].batch_size == 0)
+        return fail(BGPQ_EINVAL, "bad primitive test arguments");
     hipError_t e = rows[bi].sort(d_in, d_sizes, d_out, pad, batches, nullptr);
     if (e != hipSuccess) return fail(BGPQ_EHIP, "sort launch: %s", hipGetErrorString(e));
     HIPCHK(hipDeviceSynchronize());
@@ -412,7 +417,8 @@ int bgpq_test_merge(int key_type, int batch_size, const void* d_a, const void* d
                     int batches, int algo) {
     const int bi = batch_index(batch_size);
     const LaunchRow* rows = rows_for(key_type, 0);
-    if (bi < 0 || !rows || batches < 1) return fail(BGPQ_EINVAL, "bad primitive test arguments");
+    if (bi < 0 || !rows || batches < 1 || rows[bi].batch_size == 0 || algo < 0 || algo > 2)
+        return fail(BGPQ_EINVAL, "bad primitive test arguments");
     hipError_t e = rows[bi].merge(d_a, d_b, d_small, d_large, batches, algo, nullptr);
     if (e != hipSuccess) return fail(BGPQ_EHIP, "merge launch: %s", hipGetErrorString(e));
     HIPCHK(hipDeviceSynchronize());

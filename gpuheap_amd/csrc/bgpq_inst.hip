@@ -21,9 +21,10 @@ const LaunchRow* BGPQ_CAT(rows_u32_v, BGPQ_VAR)() {
 const LaunchRow* BGPQ_CAT(rows_u64_v, BGPQ_VAR)() {
 #endif
     static const LaunchRow rows[kNumBatchSizes] = {
-        Launch<KeyT, BGPQ_VAR, 64>::row(),  Launch<KeyT, BGPQ_VAR, 128>::row(),
-        Launch<KeyT, BGPQ_VAR, 256>::row(), Launch<KeyT, BGPQ_VAR, 512>::row(),
-        Launch<KeyT, BGPQ_VAR, 1024>::row(), Launch<KeyT, BGPQ_VAR, 2048>::row(),
+        launch_row<KeyT, BGPQ_VAR, 64>(),   launch_row<KeyT, BGPQ_VAR, 128>(),
+        launch_row<KeyT, BGPQ_VAR, 256>(),  launch_row<KeyT, BGPQ_VAR, 512>(),
+        launch_row<KeyT, BGPQ_VAR, 1024>(), launch_row<KeyT, BGPQ_VAR, 2048>(),
+        launch_row<KeyT, BGPQ_VAR, 4096>(), launch_row<KeyT, BGPQ_VAR, 8192>(),
     };
     return rows;
 }

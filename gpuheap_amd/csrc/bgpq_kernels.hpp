@@ -12,8 +12,14 @@ namespace bgpq {
 // Workgroup size per batch size.  Every thread holds B/NT keys while sorting
 // and produces 2B/NT keys per merge.
 constexpr int threads_for(int B) {
-    return B <= 128 ? 64 : B / 2;
+    return B <= 128 ? 64 : (B / 2 < 1024 ? B / 2 : 1024);
 }
+
+// gfx950 gives a workgroup up to 160 KB of LDS.  The delete path needs
+// 3*B*sizeof(K) of it: every u32 batch size up to 8192 fits, u64 up to 4096.
+constexpr int kMaxLdsBytes = 160 * 1024;
+template <typename K, int B>
+constexpr bool batch_fits() { return delete_lds_bytes<K, B>() <= kMaxLdsBytes; }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
 // XCD b mod 8).  With pack > 1 only every pack-th workgroup of the launch
@@ -99,7 +105,8 @@ __global__ __launch_bounds__(threads_for(B)) void k_prim_merge(const K* a, const
     g2l_batch<K, B, NT>(X1, b + o);
     __syncthreads();
     if constexpr (ALGO == 0) wg_merge_bitonic<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
-    else wg_merge_mp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
+    else if constexpr (ALGO == 1) wg_merge_mp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
+    else wg_merge_grp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
 }
 
 template <typename K>
@@ -143,15 +150,24 @@ inline void clear_last_error() { (void)hipGetLastError(); }
 template <typename K, int VAR, int B>
 struct Launch {
     static constexpr size_t lds = (size_t)delete_lds_bytes<K, B>();
-    static_assert(delete_lds_bytes<K, B>() <= 64 * 1024, "dynamic LDS per workgroup");
+    static_assert(batch_fits<K, B>(), "dynamic LDS per workgroup");
+    // Dynamic LDS above 64 KB must be allowed per kernel before a launch.
+    static hipError_t allow_lds(const void* fn, size_t bytes) {
+        if (bytes <= 64 * 1024) return hipSuccess;
+        return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    }
     static hipError_t insert(const HeapArgs& a, const void* keys, long long n, int chunk, int grid,
                              hipStream_t s) {
+        static const hipError_t ok = allow_lds(reinterpret_cast<const void*>(&k_insert<K, VAR, B>), lds);
+        if (ok != hipSuccess) return ok;
         clear_last_error();
         hipLaunchKernelGGL((k_insert<K, VAR, B>), dim3(grid * a.pack), dim3(threads_for(B)), lds, s, view_of<K>(a),
                            (const K*)keys, n, chunk, a.pack);
         return hipGetLastError();
     }
     static hipError_t del(const HeapArgs& a, void* out, long long calls, int grid, hipStream_t s) {
+        static const hipError_t ok = allow_lds(reinterpret_cast<const void*>(&k_delete<K, VAR, B>), lds);
+        if (ok != hipSuccess) return ok;
         clear_last_error();
         hipLaunchKernelGGL((k_delete<K, VAR, B>), dim3(grid * a.pack), dim3(threads_for(B)), lds, s, view_of<K>(a),
                            (K*)out, calls, a.pack);
@@ -159,6 +175,8 @@ struct Launch {
     }
     static hipError_t conc(const HeapArgs& a, const void* keys, long long n, int chunk, void* out,
                            long long calls, int grid, hipStream_t s) {
+        static const hipError_t ok = allow_lds(reinterpret_cast<const void*>(&k_concurrent<K, VAR, B>), lds);
+        if (ok != hipSuccess) return ok;
         clear_last_error();
         hipLaunchKernelGGL((k_concurrent<K, VAR, B>), dim3(grid * a.pack), dim3(threads_for(B)), lds, s,
                            view_of<K>(a), (const K*)keys, n, chunk, (K*)out, calls, a.pack);
@@ -172,19 +190,39 @@ struct Launch {
     }
     static hipError_t merge(const void* a, const void* b, void* small, void* large, int batches, int algo,
                             hipStream_t s) {
+        static const hipError_t ok0 =
+            allow_lds(reinterpret_cast<const void*>(&k_prim_merge<K, B, 0>), 2 * B * sizeof(K));
+        static const hipError_t ok1 =
+            allow_lds(reinterpret_cast<const void*>(&k_prim_merge<K, B, 1>), 2 * B * sizeof(K));
+        static const hipError_t ok2 =
+            allow_lds(reinterpret_cast<const void*>(&k_prim_merge<K, B, 2>), 2 * B * sizeof(K));
+        if (ok0 != hipSuccess) return ok0;
+        if (ok1 != hipSuccess) return ok1;
+        if (ok2 != hipSuccess) return ok2;
         clear_last_error();
         if (algo == 0)
             hipLaunchKernelGGL((k_prim_merge<K, B, 0>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
                                (const K*)a, (const K*)b, (K*)small, (K*)large);
-        else
+        else if (algo == 1)
             hipLaunchKernelGGL((k_prim_merge<K, B, 1>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
+                               (const K*)a, (const K*)b, (K*)small, (K*)large);
+        else
+            hipLaunchKernelGGL((k_prim_merge<K, B, 2>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
                                (const K*)a, (const K*)b, (K*)small, (K*)large);
         return hipGetLastError();
     }
     static LaunchRow row() { return LaunchRow{B, &sort, &merge, &insert, &del, &conc}; }
 };
 
-constexpr int kNumBatchSizes = 6;  // 64 .. 2048
+// A batch size whose delete workspace does not fit in LDS: an empty row
+// (batch_size 0), which bgpq_create rejects.
+template <typename K, int VAR, int B>
+LaunchRow launch_row() {
+    if constexpr (batch_fits<K, B>()) return Launch<K, VAR, B>::row();
+    else return LaunchRow{0, nullptr, nullptr, nullptr, nullptr, nullptr};
+}
+
+constexpr int kNumBatchSizes = 8;  // 64 .. 8192
 inline int batch_index(int B) {
     switch (B) {
         case 64: return 0;
@@ -193,6 +231,8 @@ inline int batch_index(int B) {
         case 512: return 3;
         case 1024: return 4;
         case 2048: return 5;
+        case 4096: return 6;
+        case 8192: return 7;
         default: return -1;
     }
 }

@@ -97,8 +97,12 @@ constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 template <typename K> __device__ __forceinline__ K kmin(K a, K b) { return b < a ? b : a; }
 template <typename K> __device__ __forceinline__ K kmax(K a, K b) { return b < a ? a : b; }
 
-// xor-lane exchange inside a wave64.  Distances 1..16 use ds_swizzle in
-// bit mode (and=0x1f, xor=m), 32 uses ds_bpermute.
+// xor-lane exchange inside a wave64.  Distances 1 and 2 use DPP quad_perm,
+// 4..16 ds_swizzle in bit mode (and=0x1f, xor=m), 32 ds_bpermute.  With
+// 1024-thread workgroups the sort and merge networks are VALU-bound (a
+// wave64 op occupies a 16-lane SIMD for 4 cycles), and these one-instruction
+// forms measured faster there than the DPP row-rotate and v_permlane*_swap
+// sequences, which need a select per exchange.
 template <int M> __device__ __forceinline__ uint32_t xlane(uint32_t v) {
     static_assert(M >= 1 && M <= 32 && (M & (M - 1)) == 0, "xor distance");
     if constexpr (M == 1) {
@@ -407,11 +411,18 @@ __device__ __forceinline__ void wg_merge_path_fast(const K* a, const K* b, K (&o
         else hi = mid;
     }
     const int ai = lo, bi = d - lo;
+    // unconditional loads at clamped indexes, then a select: a guarded load
+    // would become a branch with its own wait per key
     K x[L], y[L];
 #pragma unroll
     for (int s = 0; s < L; ++s) {
-        x[s] = (ai + s < B) ? a[ai + s] : key_max<K>();
-        y[s] = (bi + s < B) ? b[bi + s] : key_max<K>();
+        x[s] = a[kmin(ai + s, B - 1)];
+        y[s] = b[kmin(bi + s, B - 1)];
+    }
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+        x[s] = (ai + s < B) ? x[s] : key_max<K>();
+        y[s] = (bi + s < B) ? y[s] : key_max<K>();
     }
 #pragma unroll
     for (int s = 0; s < L; ++s) out[s] = kmin(x[s], y[L - 1 - s]);
@@ -559,13 +570,135 @@ __device__ __forceinline__ void wg_merge_bitonic(K* a, K* b, const DS& small, co
     __syncthreads();
 }
 
-// The merge used by the heap.  Measured on MI355X per merge: the register
-// bitonic merge wins below B = 1024 (fixed latency, no data-dependent LDS
-// reads); from B = 1024 on, the merge path with a 4-ary diagonal search is
-// faster (B = 2048, 1024 threads: 1.1 us against 1.7 us).
+// Group merge.  The 2B outputs are cut into windows of W = G*L keys, one
+// per group of G = 16 consecutive lanes (L = 2B/NT keys per lane):
+//  1. The group finds the merge-path split of its window's first output with
+//     a cooperative (G+1)-ary search: each round, lane j probes position
+//     lo + (j+1)*n/(G+1) and a ballot counts the probes that still lie
+//     before the split (3 rounds for B <= 4096, 4 for 8192), with no barrier.
+//  2. Past the split, the window holds the W smallest of the next W keys of
+//     each input.  c[p] = min(a[ai+p], b[bi+W-1-p]) (p < W) is bitonic and
+//     holds exactly them (Batcher); log2(W) half-cleaning stages sort it.
+// Lane j of a group owns window positions p = s*G + j (s < L): every LDS
+// read and write of the group is G consecutive words, free of bank
+// conflicts; strides >= G of the cleaning stages stay in the lane, smaller
+// ones use DPP / ds_swizzle.  No step depends on a previous output, so the
+// dependent chain is the search plus one LDS round trip.
+constexpr int kMergeGroup = 16;
+
+template <typename K, int B, int NT>
+struct GroupMerge {
+    static constexpr int G = kMergeGroup;
+    static constexpr int L = 2 * B / NT;
+    static constexpr int W = G * L;
+    static_assert(NT % G == 0 && B % W == 0, "group merge shape");
+
+    // Outputs of lane threadIdx.x: window base (first output index of its
+    // group) and the lane's column j; out[s] is output base + s*G + j.
+    __device__ __forceinline__ static void run(const K* a, const K* b, K (&out)[L], int& base, int& j) {
+        const int t = threadIdx.x;
+        j = t % G;
+        const int D = (t / G) * W;
+        base = D;
+        int lo = D > B ? D - B : 0;
+        int hi = D < B ? D : B;
+        const int shift = ((int)__lane_id() / G) * G;
+        // split = first q in [lo, hi] with b[D-1-q] < a[q] (or hi)
+        while (lo < hi) {
+            const int n = hi - lo;
+            const int q = lo + ((j + 1) * n) / (G + 1);
+            const bool before = !(b[D - 1 - q] < a[q]);
+            const uint64_t m = __ballot(before);
+            const int c = __popcll((m >> shift) & ((1ull << G) - 1));
+            const int nlo = c > 0 ? lo + (c * n) / (G + 1) + 1 : lo;
+            const int nhi = c < G ? lo + ((c + 1) * n) / (G + 1) : hi;
+            lo = nlo;
+            hi = nhi;
+        }
+        const int ai = lo, bi = D - lo;
+        // unconditional loads at clamped indexes, then a select (a guarded
+        // load would become a branch with its own wait per key)
+        K x[L], y[L];
+#pragma unroll
+        for (int s = 0; s < L; ++s) {
+            const int p = s * G + j;
+            x[s] = a[kmin(ai + p, B - 1)];
+            y[s] = b[kmin(bi + W - 1 - p, B - 1)];
+        }
+#pragma unroll
+        for (int s = 0; s < L; ++s) {
+            const int p = s * G + j;
+            const K xs = ai + p < B ? x[s] : key_max<K>();
+            const K ys = bi + W - 1 - p < B ? y[s] : key_max<K>();
+            out[s] = kmin(xs, ys);
+        }
+        // strides W/2 .. G: partner position p ^ stride is column j, row s ^ (stride/G)
+#pragma unroll
+        for (int r = L / 2; r > 0; r >>= 1) {
+#pragma unroll
+            for (int s = 0; s < L; ++s) {
+                if ((s & r) == 0) {
+                    const K u = out[s], v = out[s | r];
+                    out[s] = kmin(u, v);
+                    out[s | r] = kmax(u, v);
+                }
+            }
+        }
+        // strides G/2 .. 1: partner lane j ^ m
+        xclean<G / 2>(out, j);
+    }
+
+    template <int M>
+    __device__ __forceinline__ static void xclean(K (&out)[L], int j) {
+        if constexpr (M >= 1) {
+            const bool lower = (j & M) == 0;
+#pragma unroll
+            for (int s = 0; s < L; ++s) {
+                const K w = xlane<M>(out[s]);
+                out[s] = lower ? kmin(out[s], w) : kmax(out[s], w);
+            }
+            xclean<M / 2>(out, j);
+        }
+    }
+};
+
+template <typename K, int L, int G, typename D>
+__device__ __forceinline__ void put_strided(const D& dst, int off, int j, const K (&v)[L]) {
+    if constexpr (std::is_same<D, ToNone>::value) {
+        return;
+    } else if constexpr (std::is_same<D, ToLds<K>>::value) {
+#pragma unroll
+        for (int s = 0; s < L; ++s) dst.p[off + s * G + j] = v[s];
+    } else {
+#pragma unroll
+        for (int s = 0; s < L; ++s) gstore_key(dst.p + off + s * G + j, v[s]);
+    }
+}
+
+// Merge a and b (LDS) with the group merge and write the B smallest keys to
+// `small`, the B largest to `large`.  Destinations may alias a or b.  Ends
+// with a barrier.
+template <typename K, int B, int NT, typename DS, typename DL>
+__device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& small, const DL& large) {
+    using GM = GroupMerge<K, B, NT>;
+    K out[GM::L];
+    int base, j;
+    GM::run(a, b, out, base, j);
+    __syncthreads();
+    if (base < B) put_strided<K, GM::L, GM::G>(small, base, j, out);
+    else put_strided<K, GM::L, GM::G>(large, base - B, j, out);
+    __syncthreads();
+}
+
+// The merge used by the heap, by measurement on MI355X (one workgroup,
+// LDS inputs): the register bitonic merge below B = 1024 (B = 512: 0.63 us
+// against 0.84 for the merge path), the merge path with a 4-ary split
+// search up to B = 4096 (B = 2048: 1.2 us against 1.9 bitonic and 1.6
+// group), and the group merge at B = 8192 (4.2 us against 5.7).
 template <typename K, int B, int NT, typename DS, typename DL>
 __device__ __forceinline__ void wg_merge(K* a, K* b, const DS& small, const DL& large) {
-    if constexpr (B >= 1024) wg_merge_mp<K, B, NT>(a, b, small, large);
+    if constexpr (B >= 8192) wg_merge_grp<K, B, NT>(a, b, small, large);
+    else if constexpr (B >= 1024) wg_merge_mp<K, B, NT>(a, b, small, large);
     else wg_merge_bitonic<K, B, NT>(a, b, small, large);
 }
 

@@ -39,7 +39,8 @@ extern "C" {
 typedef struct bgpq_heap bgpq_heap;
 
 /* Heap(int batchNum, int batchSize, K init_limits) -- heap/buitdd/heap.cuh:34-68,
- * heap/tditdd/heap.cuh:33-67.  batch_size is a power of two in [64, 2048].
+ * heap/tditdd/heap.cuh:33-67.  batch_size is a power of two in [64, 8192]; u64 keys stop at 4096
+ * (the delete workspace, 3*batch_size*sizeof(key), must fit gfx950's 160 KB of LDS).
  * Every slot is filled with init_limits on the device; keys inserted later
  * must be strictly below init_limits.  The heap-sort output cursor
  * (deleteCount, the reference's -DHEAP_SORT mode) is always maintained. */
@@ -104,7 +105,8 @@ int bgpq_default_grid(int batch_size);
  *   batch i = d_in[i*B .. i*B+d_sizes[i]) padded with `pad`, sorted into d_out[i*B ..].
  * bgpq_test_merge: merge of sorted d_a[i], d_b[i] (imergePath, heap/heaputil.cuh:99-158):
  *   the B smallest keys to d_small[i], the B largest to d_large[i]; algo 0 is
- *   the register bitonic merge the heap uses, 1 the LDS merge path. */
+ *   the register bitonic merge, 1 the merge path, 2 the group merge (the heap
+ *   uses the group merge). */
 int bgpq_test_sort(int key_type, int batch_size, const void* d_in, const int* d_sizes, void* d_out,
                    uint64_t pad, int batches);
 int bgpq_test_merge(int key_type, int batch_size, const void* d_a, const void* d_b, void* d_small, void* d_large,

@@ -36,9 +36,11 @@ def _state_equal(h, m, key):
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
-@pytest.mark.parametrize("B", [64, 128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("B", [64, 128, 256, 512, 1024, 2048, 4096, 8192])
 @pytest.mark.parametrize("key", ["u32", "u64"])
 def test_single_workgroup_state_parity(gpu, variant, B, key):
+    if key == "u64" and B == 8192:
+        pytest.skip("u64 keys stop at B=4096 (LDS); the rejection is tested below")
     print(f"[case] state parity {variant} B={B} {key}", flush=True)
     rng = np.random.default_rng(B * 7 + VARIANTS.index(variant))
     batch_num = 128
@@ -106,6 +108,23 @@ def test_reference_harness_matrix(gpu, variant, B):
     got, want, (ok, msg) = _harness(variant, B, 1 << 20, "u32", seed=B, grid=32)
     assert ok, msg
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("B", [2048, 4096, 8192])
+def test_large_batch_harness(gpu, variant, B):
+    """The same harness with the batch sizes that need more than the 64 KB of
+    LDS a workgroup gets by default (B=8192: 96 KB)."""
+    print(f"[case] harness large batch {variant} B={B}", flush=True)
+    got, want, (ok, msg) = _harness(variant, B, 1 << 21, "u32", seed=B + 1, grid=32)
+    assert ok, msg
+    assert np.array_equal(got, want)
+
+
+def test_u64_batch_8192_rejected():
+    # 3 * 8192 * 8 bytes of delete workspace do not fit in 160 KB of LDS
+    with pytest.raises(G.BgpqError, match="LDS"):
+        Heap(4, 8192, key="u64")
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
