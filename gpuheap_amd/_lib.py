@@ -51,7 +51,29 @@ SIGNATURES = {
     "bgpq_d2h": (c_int, [c_void_p, c_void_p, c_size_t]),
     "bgpq_sync": (c_int, []),
     "bgpq_device_name": (c_int, [ctypes.c_char_p, c_int]),
+    # include/bgpq_apps.h
+    "bgpq_sssp": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, c_int,
+                          c_void_p, c_void_p]),
+    "bgpq_astar": (c_int, [c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, c_int,
+                           ctypes.POINTER(ctypes.c_uint32), c_void_p, c_void_p]),
+    "bgpq_knapsack": (c_int, [c_void_p, ctypes.c_int32, c_void_p, c_void_p, c_u64, c_i64, c_int,
+                              ctypes.POINTER(c_u64), c_void_p, c_void_p]),
+    "bgpq_gen_rmat": (c_int, [c_int, c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, c_u64,
+                              ctypes.c_uint32, ctypes.c_uint32, c_void_p, c_void_p, c_void_p]),
+    "bgpq_gen_grid": (c_int, [c_int, c_int, ctypes.c_double, c_u64, c_void_p]),
+    "bgpq_gen_knapsack": (c_int, [c_int, c_u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_void_p,
+                                  c_void_p, ctypes.POINTER(c_u64)]),
 }
+
+
+class AppStats(ctypes.Structure):
+    """bgpq_app_stats (include/bgpq_apps.h)."""
+    _fields_ = [("popped", ctypes.c_int64), ("expanded", ctypes.c_int64), ("stale", ctypes.c_int64),
+                ("pruned", ctypes.c_int64), ("pushed", ctypes.c_int64), ("error_flags", ctypes.c_int32),
+                ("ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class BgpqError(RuntimeError):

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "bgpq_handle.hpp"
 #include "bgpq_kernels.hpp"
 
 namespace bgpq {
@@ -25,21 +26,9 @@ const LaunchRow* rows_u64_v3();
 
 using namespace bgpq;
 
-struct bgpq_heap {
-    int variant;
-    int key_type;
-    int batch_num;
-    int batch_size;
-    uint64_t init_limits;
-    void* items;
-    int* status;
-    int* ctr;
-    const LaunchRow* row;
-};
-
 static thread_local std::string g_err;
 
-static int fail(int code, const char* fmt, ...) {
+int bgpq_fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -48,12 +37,8 @@ static int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
-
-#define HIPCHK(expr)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess) return fail(BGPQ_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
+#define fail bgpq_fail
+#define HIPCHK BGPQ_HIPCHK
 
 static size_t key_bytes(int key_type) { return key_type == BGPQ_KEY_U64 ? 8 : 4; }
 
@@ -90,6 +75,7 @@ static HeapArgs args_of(const bgpq_heap* h, long long out_cap = 0) {
     return HeapArgs{h->items, h->status, h->ctr, h->init_limits, h->batch_num, 1, out_cap, env_pack()};
 }
 
+
 static int fill_items(bgpq_heap* h, hipStream_t s) {
     const long long n = (long long)(h->batch_num + 1) * h->batch_size;
     hipError_t e = h->key_type == BGPQ_KEY_U64 ? fill_u64(h->items, n, h->init_limits, s)
@@ -105,6 +91,14 @@ static int counters(bgpq_heap* h, int* c) {
     HIPCHK(hipMemcpy(raw.data(), h->ctr, ctr_bytes(), hipMemcpyDeviceToHost));
     for (int i = 0; i < CTR_WORDS; ++i) c[i] = raw[(size_t)i * kWordStride];
     return BGPQ_OK;
+}
+
+int bgpq_handle_counters(bgpq_heap* h, int* c) { return counters(h, c); }
+
+HeapArgs bgpq_handle_args(const bgpq_heap* h, int heap_sort) {
+    HeapArgs a = args_of(h, 0);
+    a.heap_sort = heap_sort;
+    return a;
 }
 
 // status words 0..count-1 gathered from the padded array
@@ -195,6 +189,7 @@ int bgpq_create(bgpq_heap** out, int variant, int key_type, int batch_num, int b
     if (key_type == BGPQ_KEY_U32 && init_limits > 0xFFFFFFFFull)
         return fail(BGPQ_EINVAL, "init_limits does not fit u32");
     bgpq_heap* h = new bgpq_heap();
+    h->app_ctl = nullptr;
     h->variant = variant;
     h->key_type = key_type;
     h->batch_num = batch_num;
@@ -229,6 +224,7 @@ void bgpq_destroy(bgpq_heap* h) {
     if (h->items) (void)hipFree(h->items);
     if (h->status) (void)hipFree(h->status);
     if (h->ctr) (void)hipFree(h->ctr);
+    if (h->app_ctl) (void)hipFree(h->app_ctl);
     delete h;
 }
 

@@ -36,6 +36,14 @@ uint64_t orc_splitmix64(uint64_t* state);
 void orc_keys_u32(uint64_t seed, int64_t n, uint32_t* out);
 void orc_keys_u64(uint64_t seed, int64_t n, uint64_t* out);
 
+/* apps.c: CPU restatements of the heap consumers */
+int orc_dijkstra(int32_t n, const int32_t* rowptr, const int32_t* col, const uint32_t* w, int32_t src,
+                 uint32_t* dist, int64_t* pops);
+int orc_astar(int32_t width, int32_t height, const uint8_t* free_cells, uint32_t* cost, int64_t* expanded);
+int orc_knapsack_dp(int32_t n, const uint32_t* weight, const uint32_t* value, uint64_t capacity, uint64_t* best);
+int orc_knapsack_bnb(int32_t n, const uint32_t* weight, const uint32_t* value, uint64_t capacity,
+                     int64_t max_nodes, int dominance, uint64_t* best, int64_t* nodes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,17 @@ def lib():
         L.orc_splitmix64.argtypes = [_u64p]
         L.orc_keys_u32.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint32)]
         L.orc_keys_u64.argtypes = [ctypes.c_uint64, ctypes.c_int64, _u64p]
+        vp = ctypes.c_void_p
+        L.orc_dijkstra.restype = ctypes.c_int
+        L.orc_dijkstra.argtypes = [ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_int64)]
+        L.orc_astar.restype = ctypes.c_int
+        L.orc_astar.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_uint32),
+                                ctypes.POINTER(ctypes.c_int64)]
+        L.orc_knapsack_dp.restype = ctypes.c_int
+        L.orc_knapsack_dp.argtypes = [ctypes.c_int32, vp, vp, ctypes.c_uint64, _u64p]
+        L.orc_knapsack_bnb.restype = ctypes.c_int
+        L.orc_knapsack_bnb.argtypes = [ctypes.c_int32, vp, vp, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int,
+                                       _u64p, ctypes.POINTER(ctypes.c_int64)]
         _lib = L
     return _lib
 
@@ -169,3 +180,49 @@ def harness_keys(seed, n):
     hi = ((k[half:] >> np.uint64(1)) | np.uint64(1 << 31)).astype(np.uint32)
     hi = np.minimum(hi, np.uint32(U32_MAX - 1))
     return np.concatenate([lo, hi])
+
+
+def _vp(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def dijkstra(rowptr, col, w, source):
+    """orc_dijkstra: uint32 distances (U32_MAX = unreachable)."""
+    n = len(rowptr) - 1
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    w = np.ascontiguousarray(w, np.uint32)
+    dist = np.empty(n, np.uint32)
+    pops = ctypes.c_int64()
+    rc = lib().orc_dijkstra(n, _vp(rowptr), _vp(col), _vp(w), source, _vp(dist), ctypes.byref(pops))
+    assert rc == 0, rc
+    return dist
+
+
+def astar(free, width, height):
+    """orc_astar: (optimal cost or U32_MAX, expanded nodes)."""
+    free = np.ascontiguousarray(free, np.uint8)
+    cost = ctypes.c_uint32()
+    exp = ctypes.c_int64()
+    rc = lib().orc_astar(width, height, _vp(free), ctypes.byref(cost), ctypes.byref(exp))
+    assert rc == 0, rc
+    return cost.value, exp.value
+
+
+def knapsack_dp(w, v, cap):
+    w = np.ascontiguousarray(w, np.uint32)
+    v = np.ascontiguousarray(v, np.uint32)
+    best = np.zeros(1, np.uint64)
+    assert lib().orc_knapsack_dp(len(w), _vp(w), _vp(v), cap, _p64(best)) == 0
+    return int(best[0])
+
+
+def knapsack_bnb(w, v, cap, dominance=True, max_nodes=1 << 26):
+    """(optimum, nodes pushed, rc); rc -3 = node limit reached."""
+    w = np.ascontiguousarray(w, np.uint32)
+    v = np.ascontiguousarray(v, np.uint32)
+    best = np.zeros(1, np.uint64)
+    nodes = ctypes.c_int64()
+    rc = lib().orc_knapsack_bnb(len(w), _vp(w), _vp(v), cap, max_nodes, 1 if dominance else 0, _p64(best),
+                                ctypes.byref(nodes))
+    return int(best[0]), nodes.value, rc
