@@ -125,6 +125,119 @@ def cpu_baseline(key, sample_log2=26, pq_log2=24):
             "pq_keys_per_s": j["pq_keys_per_s"], "ok": j["ok"], "host_threads_available": os.cpu_count()}
 
 
+def _orc():
+    """The CPU oracle library (oracle/_build/liborc.so), for the baseline leg."""
+    path = os.path.join(REPO, "oracle", "_build", "liborc.so")
+    if not os.path.exists(path):
+        subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True, capture_output=True)
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.orc_dijkstra.argtypes = [ctypes.c_int32, vp, vp, vp, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_int64)]
+    L.orc_astar.argtypes = [ctypes.c_int32, ctypes.c_int32, vp, ctypes.POINTER(ctypes.c_uint32),
+                            ctypes.POINTER(ctypes.c_int64)]
+    L.orc_knapsack_dp.argtypes = [ctypes.c_int32, vp, vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+    return L
+
+
+def _vp(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def app_bench(args):
+    """One driver solve at the configured size, timed with HIP events inside
+    the library, next to the CPU oracle on the same input (one thread)."""
+    from gpuheap_amd import apps as A
+    B = args.batch
+    reps = max(args.steps, 1)
+    if args.app == "sssp":
+        scale = args.scale
+        t0 = time.perf_counter()
+        rowptr, col, w = A.rmat_csr(scale, 16, 0)
+        print(f"[bench] R-MAT scale {scale}: {len(col)} edges generated in {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+        g = A.Graph(rowptr, col, w)
+        h = Heap(batch_num_for(len(col), B), B, variant=args.variant, key="u64")
+        for _ in range(args.warmup):
+            A.sssp(h, g, 0, grid=args.grid_delete)
+        ms = []
+        for _ in range(reps):
+            dist, st = A.sssp(h, g, 0, grid=args.grid_delete)
+            ms.append(st["ms"])
+        L = _orc()
+        ref = np.empty(len(rowptr) - 1, np.uint32)
+        pops = ctypes.c_int64()
+        t0 = time.perf_counter()
+        L.orc_dijkstra(len(rowptr) - 1, _vp(rowptr), _vp(col), _vp(w), 0, _vp(ref), ctypes.byref(pops))
+        cpu_s = time.perf_counter() - t0
+        ok = bool(np.array_equal(dist, ref))
+        t = float(np.median(ms)) / 1e3
+        line = {"metric": "SSSP solve (R-MAT, Dijkstra frontier on the heap)", "value": 1.0 / t, "unit": "solves/s",
+                "ms_per_solve": t * 1e3, "edges_per_s": len(col) / t, "n_gpus": 1, "steps": reps,
+                "warmup": args.warmup, "higher_is_better": True, "dtype": "u64",
+                "config": {"workload": f"SSSP on R-MAT scale {scale} (a,b,c = 0.57,0.19,0.19), degree 16, "
+                                       f"weights 1..1000, source 0", "batch_size": B, "variant": args.variant,
+                           "grid": args.grid_delete},
+                "stats": st, "check": "distances equal CPU Dijkstra: " + ("ok" if ok else "FAILED"),
+                "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "solves/s", "cores": 1, "kind": "port",
+                                 "sample": f"binary-heap Dijkstra (oracle/apps.c) on the same graph, one solve, "
+                                           f"{cpu_s:.2f} s"}}
+    elif args.app == "astar":
+        size = args.size
+        free = A.grid_map(size, size, 0.25, 0)
+        gm = A.GridMap(free, size, size)
+        h = Heap(batch_num_for(4 * size * size, B), B, variant=args.variant, key="u64")
+        for _ in range(args.warmup):
+            A.astar(h, gm, grid=args.grid_delete)
+        ms = []
+        for _ in range(reps):
+            cost, st = A.astar(h, gm, grid=args.grid_delete)
+            ms.append(st["ms"])
+        L = _orc()
+        c = ctypes.c_uint32()
+        exp = ctypes.c_int64()
+        t0 = time.perf_counter()
+        L.orc_astar(size, size, _vp(free), ctypes.byref(c), ctypes.byref(exp))
+        cpu_s = time.perf_counter() - t0
+        ok = cost == c.value
+        t = float(np.median(ms)) / 1e3
+        line = {"metric": "A* solve (8-connected grid, octile heuristic)", "value": 1.0 / t, "unit": "solves/s",
+                "ms_per_solve": t * 1e3, "optimal_cost": cost, "cpu_expanded": exp.value, "n_gpus": 1,
+                "steps": reps, "warmup": args.warmup, "higher_is_better": True, "dtype": "u64",
+                "config": {"workload": f"A* on a {size}x{size} grid, obstacles p=0.25 seed 0, costs 10/14",
+                           "batch_size": B, "variant": args.variant, "grid": args.grid_delete},
+                "stats": st, "check": "cost equals CPU A*: " + ("ok" if ok else "FAILED"),
+                "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "solves/s", "cores": 1, "kind": "port",
+                                 "sample": f"sequential A* (oracle/apps.c) on the same grid, one solve, "
+                                           f"{cpu_s:.2f} s"}}
+    else:
+        n = args.items
+        w, v, cap = A.knapsack_instance(n, 0)
+        h = Heap(1 << 12, B, variant=args.variant, key="u64")
+        for _ in range(args.warmup):
+            A.knapsack(h, w, v, cap, grid=args.grid_delete)
+        ms = []
+        for _ in range(reps):
+            best, st = A.knapsack(h, w, v, cap, grid=args.grid_delete)
+            ms.append(st["ms"])
+        L = _orc()
+        ref = ctypes.c_uint64()
+        t0 = time.perf_counter()
+        L.orc_knapsack_dp(n, _vp(w), _vp(v), cap, ctypes.byref(ref))
+        cpu_s = time.perf_counter() - t0
+        ok = best == ref.value
+        t = float(np.median(ms)) / 1e3
+        line = {"metric": "0/1 knapsack branch-and-bound solve", "value": 1.0 / t, "unit": "solves/s",
+                "ms_per_solve": t * 1e3, "optimum": best, "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+                "higher_is_better": True, "dtype": "u64",
+                "config": {"workload": f"knapsack n={n}, weights 1..1000, values = weight + 100, capacity half",
+                           "batch_size": B, "variant": args.variant, "grid": args.grid_delete},
+                "stats": st, "check": "optimum equals CPU DP: " + ("ok" if ok else "FAILED"),
+                "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "solves/s", "cores": 1, "kind": "port",
+                                 "sample": f"O(n*C) dynamic program (oracle/apps.c), one solve, {cpu_s * 1e3:.1f} ms"}}
+    print(json.dumps(line), flush=True)
+    return 0 if ok else 1
+
+
 def traffic_for(workload):
     """HBM bytes per launch of the dominant kernel, from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/hbm_traffic.json)."""
@@ -154,6 +267,11 @@ def main():
     ap.add_argument("--sweep", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device copy (HBM counter calibration under rocprofv3)")
+    ap.add_argument("--app", choices=["sssp", "astar", "knapsack"],
+                    help="time one driver solve instead of the heap sort (--batch, --variant, --grid-delete apply)")
+    ap.add_argument("--scale", type=int, default=22, help="SSSP: log2 of the vertex count")
+    ap.add_argument("--size", type=int, default=4096, help="A*: grid side")
+    ap.add_argument("--items", type=int, default=200, help="knapsack: items")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -161,6 +279,12 @@ def main():
               "(the multi-GPU sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     print(f"[bench] device: {G.device_name()}", file=sys.stderr, flush=True)
+    if args.app:
+        if args.variant not in ("buitdd", "tditdd"):
+            args.variant = "buitdd"
+        if args.batch > 2048:
+            args.batch = 1024
+        sys.exit(app_bench(args))
 
     if args.sweep:
         for key, order in (("u32", "random"), ("u32", "descending"), ("u64", "random")):

@@ -63,6 +63,18 @@ SIGNATURES = {
     "bgpq_gen_grid": (c_int, [c_int, c_int, ctypes.c_double, c_u64, c_void_p]),
     "bgpq_gen_knapsack": (c_int, [c_int, c_u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, c_void_p,
                                   c_void_p, ctypes.POINTER(c_u64)]),
+    # include/bgpq_serve.h
+    "bgpq_now": (ctypes.c_double, []),
+    "bgpq_query_bytes": (c_i64, [c_void_p]),
+    "bgpq_server_create": (c_int, [ctypes.POINTER(c_void_p), c_i64, c_int, c_int]),
+    "bgpq_server_destroy": (None, [c_void_p]),
+    "bgpq_server_start": (c_int, [c_void_p]),
+    "bgpq_server_submit": (c_int, [c_void_p, c_void_p]),
+    "bgpq_server_poll": (c_int, [c_void_p, c_void_p, c_int]),
+    "bgpq_server_inflight": (c_int, [c_void_p]),
+    "bgpq_server_used_bytes": (c_i64, [c_void_p]),
+    "bgpq_server_close": (c_int, [c_void_p]),
+    "bgpq_batch_run": (c_int, [c_void_p, c_int, c_i64, c_int, c_int, c_void_p, c_float_p]),
 }
 
 

@@ -37,7 +37,8 @@ enum {
     APP_ERROR = 6,
     APP_BEST = 7,  // knapsack incumbent
     APP_POOL = 8,  // knapsack node-pool cursor
-    APP_WORDS = 9
+    APP_DONE = 9,  // team members that have left the solve
+    APP_WORDS = 10
 };
 constexpr int kCtlStride = 16;  // long longs per control word
 enum { APPERR_RANGE = 1, APPERR_POOL = 2, APPERR_IDLE = 4, APPERR_HEAP = 8 };
@@ -64,6 +65,9 @@ __device__ __forceinline__ uint32_t u32_min(uint32_t* p, uint32_t v) {
 }
 __device__ __forceinline__ uint32_t u32_max(uint32_t* p, uint32_t v) {
     return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t u64_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Exclusive prefix sum over the workgroup; `tmp` holds NT/64 + 1 ints.
@@ -106,10 +110,13 @@ struct AppStats {
     int popped, expanded, stale, pruned, pushed;
 };
 
-template <int VAR, int B, class App>
-__device__ void drive(HeapView<uint64_t> h, const App& app, long long* ctl, uint64_t first_key) {
+// One workgroup's part in a solve.  Any number of workgroups may run it on
+// the same heap and control words (a team); the one with `first` set
+// inserts the first key, which the outstanding count already includes.
+template <int VAR, int B, int NT, class App>
+__device__ void drive(HeapView<uint64_t> h, const App& app, long long* ctl, uint64_t first_key, bool first) {
     using K = uint64_t;
-    constexpr int NT = threads_for(B);
+    static_assert(NT <= B && B % NT == 0, "workgroup shape");
     constexpr int P = B / NT;
     constexpr int kWs = app_ws_offset<B>();
     static_assert(kWs % 4 == 0, "heap workspace offset must be 16-byte aligned");
@@ -128,8 +135,7 @@ __device__ void drive(HeapView<uint64_t> h, const App& app, long long* ctl, uint
         bcast_slots()[BC_ABORT] = 0;
     }
     __syncthreads();
-    // workgroup 0 inserts the first key; outstanding already counts it
-    if (blockIdx.x == 0) {
+    if (first) {
         if (t == 0) popped[0] = first_key;
         __syncthreads();
         hp.insertion(popped, 1, kWs);
@@ -336,7 +342,9 @@ struct AstarApp {
 // node survives only if its value beats the best recorded for its
 // (depth, used weight) cell, and a popped node whose cell has since
 // improved is stale.  Nodes are 16-byte records in a pool; the key's low
-// word is the record index.
+// word is the record index.  Every array of the solve is read with
+// device-coherent loads: a long-running serving kernel may meet them in
+// memory it has cached under an earlier query.
 struct KnapNode {
     uint32_t level, used, value, pad;
 };
@@ -365,14 +373,15 @@ struct KnapsackApp {
     // value + whole items from `level` while they fit + floor(room * v / w)
     // of the first that does not (binary search on the prefix sums)
     __device__ uint64_t bound(uint32_t level, uint64_t room, uint64_t value) const {
+        const uint64_t base_w = u64_load(pw + level);
         int lo = (int)level, hi = n;  // largest m with pw[m] - pw[level] <= room
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (pw[mid] - pw[level] <= room) lo = mid;
+            if (u64_load(pw + mid) - base_w <= room) lo = mid;
             else hi = mid - 1;
         }
-        uint64_t ub = value + (pv[lo] - pv[level]);
-        if (lo < n) ub += (room - (pw[lo] - pw[level])) * v[lo] / w[lo];
+        uint64_t ub = value + (u64_load(pv + lo) - u64_load(pv + level));
+        if (lo < n) ub += (room - (u64_load(pw + lo) - base_w)) * u32_load(v + lo) / u32_load(w + lo);
         return ub;
     }
     __device__ uint32_t* slot(uint32_t level, uint64_t used) const {
@@ -397,9 +406,10 @@ struct KnapsackApp {
         const uint32_t lv = nd.level;
         uint64_t used = nd.used, value = nd.value;
         if (j == 0) {  // take item lv
-            if (used + w[lv] > cap) return false;
-            used += w[lv];
-            value += v[lv];
+            const uint32_t wl = u32_load(w + lv);
+            if (used + wl > cap) return false;
+            used += wl;
+            value += u32_load(v + lv);
         }
         const uint32_t cl = lv + 1;
         if ((long long)value > ctl_load(ctl, APP_BEST))

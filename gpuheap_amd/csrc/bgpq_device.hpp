@@ -45,10 +45,12 @@ enum { ERR_OVERFLOW = 1, ERR_BAD_SIZE = 2, ERR_LOST_KEYS = 4, ERR_SPIN_LIMIT = 8
 constexpr int kWordStride = 32;
 
 // A lock wait longer than this many polls (each at least one device-scope
-// atomic, i.e. seconds in total) trips the heap's abort word: every
-// workgroup then leaves its current operation at the next hand-off point, so
-// a protocol fault ends the kernel with ERR_SPIN_LIMIT instead of hanging.
-constexpr int kSpinLimit = 1 << 22;
+// load plus a sleep, so tens of seconds in total) trips the heap's abort
+// word: every workgroup then leaves its current operation at the next
+// hand-off point, so a protocol fault ends the kernel with ERR_SPIN_LIMIT
+// instead of hanging.  The locks are not fair: with hundreds of workgroups
+// on status[0] a waiter can lose for seconds, which must not count as a fault.
+constexpr int kSpinLimit = 1 << 25;
 
 template <typename K>
 struct HeapView {

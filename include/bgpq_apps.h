@@ -41,7 +41,7 @@ typedef struct bgpq_app_stats {
 /* Dijkstra with the heap as frontier: keys are dist << 32 | vertex (the
  * reference's MAKE_KEY, sssp.cu:6).  CSR with int32 indices, u32 weights;
  * d_dist[n_vertices] receives the distances (UINT32_MAX = unreachable).
- * grid 0 picks the default.  Fails with BGPQ_EHEAP if a distance would not
+ * grid is the number of workgroups (0: 64).  Fails with BGPQ_EHEAP if a distance would not
  * fit in 32 bits or the heap overflowed. */
 int bgpq_sssp(bgpq_heap* h, const int32_t* d_rowptr, const int32_t* d_col, const uint32_t* d_w,
               int32_t n_vertices, int32_t source, uint32_t* d_dist, int grid, bgpq_app_stats* stats,

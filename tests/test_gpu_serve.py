@@ -1,0 +1,136 @@
+"""GPU tests of the batched and served solves (include/bgpq_serve.h).
+
+* A mixed batch (SSSP, A*, knapsack; several batch sizes and team sizes)
+  through the captured hipGraph, replayed: every result equals its CPU oracle
+  (SSSP distances bit for bit, A* costs, knapsack optima).
+* The persistent serve kernel: each query gives the same result alone (the
+  only query in flight) and in a mix submitted all at once, and both equal
+  the oracle.
+* Admission: with a budget that holds one query at a time, submissions
+  wait (EAGAIN) until completions free the arena, and all complete.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from gpuheap_amd import apps as A
+from gpuheap_amd import serve as S
+
+pytestmark = pytest.mark.gpu
+
+GiB = 1 << 30
+
+
+def _catalog():
+    graphs = [(A.Graph(*A.rmat_csr(s, 16, s)), A.rmat_csr(s, 16, s)) for s in (8, 10, 12)]
+    grids = []
+    for size, seed in ((64, 0), (96, 1), (128, 2)):
+        free = A.grid_map(size, size, 0.25, seed)
+        grids.append((A.GridMap(free, size, size), free, size))
+    knaps = [A.knapsack_instance(n, seed) for n, seed in ((30, 0), (60, 1), (100, 2))]
+    return graphs, grids, knaps
+
+
+def _queries(graphs, grids, knaps, with_labels=True):
+    qs, want = [], []
+    tag = 0
+    for (g, (rp, col, w)), B, team in zip(graphs, (256, 512, 1024), (1, 2, 4)):
+        for src in (0, 3):
+            out = np.zeros(g.n, np.uint32) if with_labels else None
+            qs.append(S.sssp_query(g, src, batch_size=B, team=team, tag=tag, labels_out=out))
+            dist = O.dijkstra(rp, col, w, src)
+            want.append(("sssp", S.sssp_digest(dist), int(np.count_nonzero(dist != O.U32_MAX)), out, dist))
+            tag += 1
+    for (gm, free, size), B, team in zip(grids, (256, 512, 256), (1, 2, 3)):
+        qs.append(S.astar_query(gm, batch_size=B, team=team, tag=tag))
+        want.append(("astar", None, O.astar(free, size, size)[0], None, None))
+        tag += 1
+    for (w, v, cap), B, team in zip(knaps, (256, 1024, 512), (1, 2, 2)):
+        qs.append(S.knapsack_query(w, v, cap, batch_size=B, team=team, tag=tag))
+        want.append(("knapsack", None, O.knapsack_dp(w, v, cap), None, None))
+        tag += 1
+    return qs, want
+
+
+def _check(res, want):
+    kind, digest, value, out, dist = want
+    assert res["status"] == 0, res
+    assert res["value"] == value, (kind, res["value"], value)
+    if kind == "sssp":
+        assert res["digest"] == digest
+        if out is not None:
+            assert np.array_equal(out, dist)
+
+
+def test_batch_graph_mixed(gpu):
+    print("[case] batch graph mixed", flush=True)
+    graphs, grids, knaps = _catalog()
+    qs, want = _queries(graphs, grids, knaps)
+    res, ms = S.batch_run(qs, 4 * GiB, workgroups=64, repeats=2)
+    assert ms > 0
+    for r, w in zip(res, want):
+        _check(r, w)
+
+
+def _drain(srv, n, timeout=60.0):
+    got = {}
+    t0 = time.time()
+    while len(got) < n:
+        for r in srv.poll():
+            got[r["tag"]] = r
+        assert time.time() - t0 < timeout, f"{n - len(got)} queries still running"
+        time.sleep(0.001)
+    return got
+
+
+def test_server_alone_vs_mixed(gpu):
+    print("[case] server alone vs mixed", flush=True)
+    graphs, grids, knaps = _catalog()
+    alone_q, want = _queries(graphs, grids, knaps, with_labels=False)
+    with S.Server(4 * GiB, workgroups=64) as srv:
+        alone = {}
+        for q in alone_q:
+            assert srv.submit(q)
+            alone.update(_drain(srv, 1))
+        mixed_q, _ = _queries(graphs, grids, knaps, with_labels=False)
+        for q in mixed_q:
+            while not srv.submit(q):
+                time.sleep(0.001)
+        mixed = _drain(srv, len(mixed_q))
+    for tag, w in enumerate(want):
+        _check(alone[tag], w)
+        _check(mixed[tag], w)
+        assert (alone[tag]["value"], alone[tag]["digest"]) == (mixed[tag]["value"], mixed[tag]["digest"])
+
+
+def test_server_budget_admission(gpu):
+    print("[case] server admission", flush=True)
+    graphs, grids, knaps = _catalog()
+    qs, want = _queries(graphs, grids, knaps, with_labels=False)
+    big = max(S.query_bytes(q) for q in qs)
+    with S.Server(big + (1 << 20), workgroups=32) as srv:
+        waited = 0
+        got = {}
+        for q in qs:
+            while not srv.submit(q):
+                waited += 1
+                for r in srv.poll():
+                    got[r["tag"]] = r
+                time.sleep(0.0005)
+            assert srv.used_bytes() <= big + (1 << 20)
+        got.update(_drain(srv, len(qs) - len(got)))
+    assert waited > 0
+    for tag, w in enumerate(want):
+        _check(got[tag], w)
+
+
+def test_server_rejects_bad_queries(gpu):
+    import gpuheap_amd as G
+    w, v, cap = A.knapsack_instance(10, 0)
+    with S.Server(1 << 28, workgroups=4) as srv:
+        with pytest.raises(G.BgpqError, match="batch size"):
+            srv.submit(S.knapsack_query(w, v, cap, batch_size=2048))
+        with pytest.raises(G.BgpqError, match="team"):
+            srv.submit(S.knapsack_query(w, v, cap, team=0))
