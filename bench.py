@@ -238,6 +238,97 @@ def app_bench(args):
     return 0 if ok else 1
 
 
+def mixed_bench(args):
+    """The mixed SSSP / A* / knapsack stream: the plain one-heap-per-launch
+    path, then the persistent serve kernel closed-loop (everything arrives at
+    once: solves/s) and at a Poisson rate (p50 / p99 latency)."""
+    from gpuheap_amd import mixed as M
+    t0 = time.perf_counter()
+    cat = M.Catalog(scales=tuple(range(10, args.max_scale + 1)), sides=tuple(
+        s for s in M.ASTAR_SIDES if s <= args.max_side))
+    print(f"[bench] catalog on the device ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    stream = M.make_stream(cat, args.queries)
+    print(f"[bench] catalog and a {len(stream)}-query stream ready in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    budget = int(args.budget_gib * (1 << 30))
+    tpath = os.path.join(REPO, "profiles", "autotune_mixed.json")
+    if os.path.exists(tpath) and not args.retune:
+        table = M.load_table(tpath)
+    else:
+        table = M.autotune(cat, budget, log=lambda m: print(m, file=sys.stderr, flush=True))
+        if args.retune:
+            M.save_table(table, os.path.join(REPO, "gpurun_out", "autotune_mixed.json")
+                         if os.path.isdir(os.path.join(REPO, "gpurun_out")) else tpath)
+    print(f"[bench] size-class table: {json.dumps(table)}", file=sys.stderr, flush=True)
+    plain, plain_lat, plain_s = M.run_plain(cat, stream, log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(f"[bench] plain path: {len(stream) / plain_s:.1f} q/s", file=sys.stderr, flush=True)
+    served, _, makespan = M.run_served(cat, stream, table, rate=0.0, budget=budget)
+    qps = len(stream) / makespan
+    print(f"[bench] served closed-loop: {qps:.1f} q/s", file=sys.stderr, flush=True)
+    rate = args.rate if args.rate > 0 else 0.5 * qps
+    served_r, lat, _ = M.run_served(cat, stream, table, rate=rate, budget=budget)
+    same = all(M.answer_of(k, key, a) == M.answer_of(k, key, b) and b["status"] == 0
+               for (k, key), a, b in zip(stream, plain, served))
+    same = same and all(M.answer_of(k, key, a) == M.answer_of(k, key, b) and b["status"] == 0
+                        for (k, key), a, b in zip(stream, plain, served_r))
+    line = {"metric": "mixed SSSP/A*/knapsack stream: solves/s (persistent serve kernel) and p99 latency",
+            "value": qps, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": makespan * 1e3, "higher_is_better": True, "dtype": "u64",
+            "config": {"workload": f"{len(stream)} queries: 40% SSSP on R-MAT 2^10..2^{args.max_scale}, 40% A* on "
+                                   f"256^2..{args.max_side}^2 grids (p=0.25), 20% knapsack n=200",
+                       "hbm_budget_gib": args.budget_gib, "size_class_table": table},
+            "plain_one_heap_per_launch": {"value": len(stream) / plain_s, "unit": "solves/s",
+                                          "p50_ms": float(np.percentile(plain_lat, 50) * 1e3),
+                                          "p99_ms": float(np.percentile(plain_lat, 99) * 1e3)},
+            "speedup_vs_plain": qps / (len(stream) / plain_s),
+            "served_at_rate": {"rate_qps": rate, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                               "p99_ms": float(np.percentile(lat, 99) * 1e3)},
+            "check": "served results equal the plain path's: " + ("ok" if same else "FAILED")}
+    if not args.no_cpu:
+        line["cpu_baseline"], ok = mixed_cpu_baseline(cat, stream, plain)
+        line["check"] += "; plain results equal the CPU oracle: " + ("ok" if ok else "FAILED")
+        same = same and ok
+    print(json.dumps(line), flush=True)
+    return 0 if same else 1
+
+
+def mixed_cpu_baseline(cat, stream, plain):
+    """CPU oracle on every distinct problem of the stream (one thread): its
+    time, scaled to the stream, is the CPU baseline, and its answers check
+    the GPU results."""
+    from gpuheap_amd import serve as S
+    L = _orc()
+    answers, secs = {}, {}
+    for kind, key in sorted(set(stream), key=str):
+        t0 = time.perf_counter()
+        if kind == "sssp":
+            rowptr, col, w = cat.graphs[key[0]]["csr"]
+            dist = np.empty(len(rowptr) - 1, np.uint32)
+            pops = ctypes.c_int64()
+            L.orc_dijkstra(len(rowptr) - 1, _vp(rowptr), _vp(col), _vp(w), key[1], _vp(dist), ctypes.byref(pops))
+            ans = (int(np.count_nonzero(dist != 0xFFFFFFFF)), S.sssp_digest(dist))
+        elif kind == "astar":
+            g = cat.grids[key]
+            c = ctypes.c_uint32()
+            exp = ctypes.c_int64()
+            side = g["dev"].width
+            L.orc_astar(side, side, _vp(g["free"]), ctypes.byref(c), ctypes.byref(exp))
+            ans = (c.value, 0)
+        else:
+            w, v, cap = cat.knaps[key]
+            best = ctypes.c_uint64()
+            L.orc_knapsack_dp(len(w), _vp(w), _vp(v), cap, ctypes.byref(best))
+            ans = (best.value, 0)
+        secs[(kind, key)] = time.perf_counter() - t0
+        answers[(kind, key)] = ans
+    from gpuheap_amd import mixed as M
+    ok = all(M.answer_of(k, key, r) == answers[(k, key)] for (k, key), r in zip(stream, plain))
+    total = sum(secs[q] for q in stream)
+    return {"value": len(stream) / total, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"{len(answers)} distinct problems solved once each (Dijkstra, A*, knapsack DP; "
+                      f"oracle/apps.c), time scaled to the {len(stream)}-query stream"}, ok
+
+
 def traffic_for(workload):
     """HBM bytes per launch of the dominant kernel, from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/hbm_traffic.json)."""
@@ -267,7 +358,13 @@ def main():
     ap.add_argument("--sweep", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device copy (HBM counter calibration under rocprofv3)")
-    ap.add_argument("--app", choices=["sssp", "astar", "knapsack"],
+    ap.add_argument("--queries", type=int, default=2000, help="mixed: queries in the stream")
+    ap.add_argument("--max-scale", type=int, default=20, help="mixed: largest SSSP scale")
+    ap.add_argument("--max-side", type=int, default=2048, help="mixed: largest A* grid side")
+    ap.add_argument("--budget-gib", type=float, default=64.0, help="mixed: HBM budget of the serve arena")
+    ap.add_argument("--rate", type=float, default=0.0, help="mixed: Poisson rate (0: half the closed-loop rate)")
+    ap.add_argument("--retune", action="store_true", help="mixed: run the autotuner instead of the committed table")
+    ap.add_argument("--app", choices=["sssp", "astar", "knapsack", "mixed"],
                     help="time one driver solve instead of the heap sort (--batch, --variant, --grid-delete apply)")
     ap.add_argument("--scale", type=int, default=22, help="SSSP: log2 of the vertex count")
     ap.add_argument("--size", type=int, default=4096, help="A*: grid side")
@@ -279,6 +376,8 @@ def main():
               "(the multi-GPU sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     print(f"[bench] device: {G.device_name()}", file=sys.stderr, flush=True)
+    if args.app == "mixed":
+        sys.exit(mixed_bench(args))
     if args.app:
         if args.variant not in ("buitdd", "tditdd"):
             args.variant = "buitdd"
