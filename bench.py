@@ -71,20 +71,30 @@ def fingerprint(a):
                 int(np.sum(x * x, dtype=np.uint64)))
 
 
-def run_config(n, key, variant, batch, gi, gd, steps, warmup, order="random", seed=0, verify=True):
+def make_input(n, key, order="random", seed=0):
+    """Device buffer of n keys: SplitMix64 keys generated on the device, in
+    generation order or sorted ascending / descending (the adversarial order
+    of artifacts/synthetic/sort.cu:115-119)."""
     kt, dtype, ks = KEYS[key]
     d_in = DeviceBuffer(n * ks)
-    d_out = DeviceBuffer(n * ks)
-    if order == "random":
-        check(lib().bgpq_gen_keys(kt, seed, n, d_in.ptr, None))
-    else:
-        check(lib().bgpq_gen_keys(kt, seed, n, d_in.ptr, None))
+    check(lib().bgpq_gen_keys(kt, seed, n, d_in.ptr, None))
+    if order != "random":
         host = np.sort(d_in.to_numpy(dtype, n))
         if order == "descending":
             host = host[::-1].copy()
         d_in.upload(host)
         del host
     G.sync()
+    return d_in
+
+
+def run_config(n, key, variant, batch, gi, gd, steps, warmup, order="random", seed=0, verify=True, d_in=None,
+               fp_in=None):
+    kt, dtype, ks = KEYS[key]
+    own_in = d_in is None
+    if own_in:
+        d_in = make_input(n, key, order, seed)
+    d_out = DeviceBuffer(n * ks)
     h = Heap(batch_num_for(n, batch), batch, variant=variant, key=key)
     for _ in range(warmup):
         h.heapsort(d_in, n, grid_insert=gi, grid_delete=gd, d_out=d_out)
@@ -99,12 +109,16 @@ def run_config(n, key, variant, batch, gi, gd, steps, warmup, order="random", se
     wall = time.perf_counter() - t0
     ok = None
     if verify:
-        src = d_in.to_numpy(dtype, n)
+        if fp_in is None:
+            src = d_in.to_numpy(dtype, n)
+            fp_in = fingerprint(src)
+            del src
         out = d_out.to_numpy(dtype, n)
-        ok = bool(np.all(out[1:] >= out[:-1])) and fingerprint(src) == fingerprint(out)
-        del src, out
+        ok = bool(np.all(out[1:] >= out[:-1])) and fp_in == fingerprint(out)
+        del out
     h.close()
-    d_in.free()
+    if own_in:
+        d_in.free()
     d_out.free()
     t_ins, t_del = float(np.mean(ins)), float(np.mean(dele))
     return {"ms_insert": t_ins, "ms_delete": t_del, "ms_step_wall": 1e3 * wall / max(steps, 1),
@@ -356,6 +370,8 @@ def main():
     ap.add_argument("--order", choices=["random", "ascending", "descending"], default="random")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true")
+    ap.add_argument("--sweep-sets", default="u32:random,u32:descending,u64:random",
+                    help="key sets of the sweep, key:order pairs")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device copy (HBM counter calibration under rocprofv3)")
     ap.add_argument("--queries", type=int, default=2000, help="mixed: queries in the stream")
@@ -386,14 +402,22 @@ def main():
         sys.exit(app_bench(args))
 
     if args.sweep:
-        for key, order in (("u32", "random"), ("u32", "descending"), ("u64", "random")):
+        # the heap-sort sweep: every variant and batch size on three key sets,
+        # each generated once and reused (u64 keys stop at B = 4096)
+        for key, order in (tuple(x.split(":")) for x in args.sweep_sets.split(",")):
+            d_in = make_input(args.n, key, order)
+            fp_in = fingerprint(d_in.to_numpy(KEYS[key][1], args.n))
             for variant in ("buitdd", "buitdd_np", "tditdd", "tditdd_np"):
                 for b in (256, 512, 1024, 2048, 4096, 8192):
-                    r = run_config(args.n, key, variant, b, args.grid_insert, args.grid_delete, 1, 0, order=order)
+                    if key == "u64" and b > 4096:
+                        continue
+                    r = run_config(args.n, key, variant, b, args.grid_insert, args.grid_delete, 1, 0, order=order,
+                                   d_in=d_in, fp_in=fp_in)
                     ab = alg_bytes(args.n, b, KEYS[key][2])
                     r.update({"n": args.n, "key": key, "order": order, "variant": variant, "batch": b,
                               "hbm_frac_total": ab["total"] / ((r["ms_insert"] + r["ms_delete"]) / 1e3) / 8e12})
                     print(json.dumps(r), flush=True)
+            d_in.free()
         return
 
     print(f"[bench] heap sort n={args.n} {args.key} {args.variant} B={args.batch} "

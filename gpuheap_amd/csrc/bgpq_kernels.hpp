@@ -106,7 +106,8 @@ __global__ __launch_bounds__(threads_for(B)) void k_prim_merge(const K* a, const
     __syncthreads();
     if constexpr (ALGO == 0) wg_merge_bitonic<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
     else if constexpr (ALGO == 1) wg_merge_mp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
-    else wg_merge_grp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
+    else if constexpr (ALGO == 2) wg_merge_grp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
+    else wg_merge_ser<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
 }
 
 template <typename K>
@@ -196,9 +197,12 @@ struct Launch {
             allow_lds(reinterpret_cast<const void*>(&k_prim_merge<K, B, 1>), 2 * B * sizeof(K));
         static const hipError_t ok2 =
             allow_lds(reinterpret_cast<const void*>(&k_prim_merge<K, B, 2>), 2 * B * sizeof(K));
+        static const hipError_t ok3 =
+            allow_lds(reinterpret_cast<const void*>(&k_prim_merge<K, B, 3>), 2 * B * sizeof(K));
         if (ok0 != hipSuccess) return ok0;
         if (ok1 != hipSuccess) return ok1;
         if (ok2 != hipSuccess) return ok2;
+        if (ok3 != hipSuccess) return ok3;
         clear_last_error();
         if (algo == 0)
             hipLaunchKernelGGL((k_prim_merge<K, B, 0>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
@@ -206,8 +210,11 @@ struct Launch {
         else if (algo == 1)
             hipLaunchKernelGGL((k_prim_merge<K, B, 1>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
                                (const K*)a, (const K*)b, (K*)small, (K*)large);
-        else
+        else if (algo == 2)
             hipLaunchKernelGGL((k_prim_merge<K, B, 2>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
+                               (const K*)a, (const K*)b, (K*)small, (K*)large);
+        else
+            hipLaunchKernelGGL((k_prim_merge<K, B, 3>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
                                (const K*)a, (const K*)b, (K*)small, (K*)large);
         return hipGetLastError();
     }

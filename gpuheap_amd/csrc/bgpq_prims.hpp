@@ -473,6 +473,50 @@ __device__ __forceinline__ void put_run(const D& dst, int off, const K (&v)[N]) 
     }
 }
 
+// Merge path with a serial, branch-free merge: a binary search finds the
+// thread's diagonal split, then each of its L outputs costs one compare,
+// two selects and one LDS load at a selected address (the next key of the
+// side just consumed).  About 8 VALU operations per output, against the
+// log2(2L)-stage cleaner of wg_merge_path_fast; the dependent chain is
+// longer (one LDS round trip per output), which the other waves hide.
+template <typename K, int B, int NT>
+__device__ __forceinline__ void wg_merge_path_serial(const K* a, const K* b, K (&out)[2 * B / NT]) {
+    constexpr int L = 2 * B / NT;
+    const int d = threadIdx.x * L;
+    int lo = d > B ? d - B : 0;
+    int hi = d < B ? d : B;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (!(b[d - 1 - mid] < a[mid])) lo = mid + 1;
+        else hi = mid;
+    }
+    int ai = lo, bi = d - lo;
+    K av = a[kmin(ai, B - 1)];
+    K bv = b[kmin(bi, B - 1)];
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+        const bool ta = bi >= B || (ai < B && !(bv < av));
+        out[s] = ta ? av : bv;
+        ai += ta ? 1 : 0;
+        bi += ta ? 0 : 1;
+        const K nv = ta ? a[kmin(ai, B - 1)] : b[kmin(bi, B - 1)];
+        av = ta ? nv : av;
+        bv = ta ? bv : nv;
+    }
+}
+
+template <typename K, int B, int NT, typename DS, typename DL>
+__device__ __forceinline__ void wg_merge_ser(const K* a, const K* b, const DS& small, const DL& large) {
+    constexpr int L = 2 * B / NT;
+    K out[L];
+    wg_merge_path_serial<K, B, NT>(a, b, out);
+    __syncthreads();
+    const int d = threadIdx.x * L;
+    if (d < B) put_run<K, L>(small, d, out);
+    else put_run<K, L>(large, d - B, out);
+    __syncthreads();
+}
+
 // Merge a and b (LDS) with the merge-path kernel above and write the B
 // smallest keys to `small`, the B largest to `large`.  Destinations may
 // alias a or b.  Ends with a barrier.

@@ -793,22 +793,43 @@ extern "C" int bgpq_batch_run(const bgpq_query* q, int n, int64_t hbm_budget, in
     }
     if (rc == BGPQ_OK && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
         rc = bgpq_fail(BGPQ_EHIP, "graph instantiate");
+    // Every replay is read back (an eager copy between replays) and must
+    // give the first replay's answers: a replay starts from the state the
+    // graph's own fill kernels write, never from what a previous one left.
     float total_ms = 0.f;
+    std::vector<Completion> first;
     for (int r = 0; r < repeats && rc == BGPQ_OK; ++r) {
         float ms = 0.f;
+        int fault = 0;
         if (hipEventRecord(e0, s) != hipSuccess || hipGraphLaunch(exec, s) != hipSuccess ||
             hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
             rc = bgpq_fail(BGPQ_EHIP, "graph launch: %s", hipGetErrorString(hipGetLastError()));
+            break;
+        }
         total_ms += ms;
+        if (hipMemcpy(comp.data(), ring + o_done, sizeof(Completion) * qcap, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(&fault, ring + o_fault, sizeof fault, hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = bgpq_fail(BGPQ_EHIP, "batch results");
+            break;
+        }
+        if (fault) {
+            rc = bgpq_fail(BGPQ_EHEAP, "batch serve kernel fault");
+            break;
+        }
+        if (r == 0) {
+            first = comp;
+        } else {
+            for (int i = 0; i < n; ++i) {
+                if (comp[i].value != first[i].value || comp[i].digest != first[i].digest ||
+                    comp[i].error != first[i].error) {
+                    rc = bgpq_fail(BGPQ_EHEAP, "batch replay %d differs from the first at query %d", r, i);
+                    break;
+                }
+            }
+        }
     }
     if (rc == BGPQ_OK) {
-        int fault = 0;
-        if (hipMemcpy(comp.data(), ring + o_done, sizeof(Completion) * qcap, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(&fault, ring + o_fault, sizeof fault, hipMemcpyDeviceToHost) != hipSuccess)
-            rc = bgpq_fail(BGPQ_EHIP, "batch results");
-        else if (fault)
-            rc = bgpq_fail(BGPQ_EHEAP, "batch serve kernel fault");
         if (ms_per_run) *ms_per_run = total_ms / repeats;
         for (int i = 0; i < n && rc == BGPQ_OK; ++i) {
             if (!comp[i].done) rc = bgpq_fail(BGPQ_EHEAP, "batch query %d did not complete", i);

@@ -105,8 +105,9 @@ int bgpq_default_grid(int batch_size);
  *   batch i = d_in[i*B .. i*B+d_sizes[i]) padded with `pad`, sorted into d_out[i*B ..].
  * bgpq_test_merge: merge of sorted d_a[i], d_b[i] (imergePath, heap/heaputil.cuh:99-158):
  *   the B smallest keys to d_small[i], the B largest to d_large[i]; algo 0 is
- *   the register bitonic merge, 1 the merge path, 2 the group merge (the heap
- *   uses the group merge). */
+ *   the register bitonic merge, 1 the merge path with a 4-ary split search,
+ *   2 the group merge, 3 the serial branch-free merge path (the heap picks
+ *   one of them by batch size). */
 int bgpq_test_sort(int key_type, int batch_size, const void* d_in, const int* d_sizes, void* d_out,
                    uint64_t pad, int batches);
 int bgpq_test_merge(int key_type, int batch_size, const void* d_a, const void* d_b, void* d_small, void* d_large,

@@ -103,8 +103,10 @@ int bgpq_server_close(bgpq_server* s);
 /* Offline batch: every query of the list at once (they must all fit in
  * hbm_budget), prepared and solved by one captured hipGraph
  * [prepare kernels -> persistent kernel over a closed device-side ring],
- * replayed `repeats` times; results of the last replay are written to
- * out[n].  ms_per_run receives the mean device time of one replay. */
+ * replayed `repeats` times.  Each replay's answers are copied back and must
+ * equal the first replay's (BGPQ_EHEAP otherwise); the last replay's
+ * results are written to out[n].  ms_per_run receives the mean device time
+ * of one replay. */
 int bgpq_batch_run(const bgpq_query* q, int n, int64_t hbm_budget, int workgroups, int repeats, bgpq_result* out,
                    float* ms_per_run);
 

@@ -47,7 +47,7 @@ def test_sort_batch(gpu, B, key):
 
 @pytest.mark.parametrize("B", BATCHES)
 @pytest.mark.parametrize("key", ["u32", "u64"])
-@pytest.mark.parametrize("algo", [0, 1, 2], ids=["bitonic", "merge_path", "group"])
+@pytest.mark.parametrize("algo", [0, 1, 2, 3], ids=["bitonic", "merge_path", "group", "serial"])
 def test_merge_batches(gpu, B, key, algo):
     kt, dtype = KEYS[key]
     if key == "u64" and B == 8192:

@@ -68,7 +68,7 @@ def test_batch_graph_mixed(gpu):
     print("[case] batch graph mixed", flush=True)
     graphs, grids, knaps = _catalog()
     qs, want = _queries(graphs, grids, knaps)
-    res, ms = S.batch_run(qs, 4 * GiB, workgroups=64, repeats=2)
+    res, ms = S.batch_run(qs, 4 * GiB, workgroups=64, repeats=4)
     assert ms > 0
     for r, w in zip(res, want):
         _check(r, w)
