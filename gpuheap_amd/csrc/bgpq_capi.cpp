@@ -197,11 +197,7 @@ int bgpq_create(bgpq_heap** out, int variant, int key_type, int batch_num, int b
     h->init_limits = init_limits;
     h->row = &rows_for(key_type, variant)[bi];
     const size_t bytes = key_bytes(key_type) * (size_t)batch_size * (size_t)(batch_num + 1);
-    const char* mode = getenv("BGPQ_ALLOC");
-    const bool fine = mode && !strcmp(mode, "fine");
-    auto alloc = [&](void** p, size_t n) {
-        return fine ? hipExtMallocWithFlags(p, n, hipDeviceMallocFinegrained) : hipMalloc(p, n);
-    };
+    auto alloc = [](void** p, size_t n) { return hipMalloc(p, n); };
     hipError_t e = alloc(&h->items, bytes);
     if (e == hipSuccess) e = alloc((void**)&h->status, status_bytes(batch_num));
     if (e == hipSuccess) e = alloc((void**)&h->ctr, ctr_bytes());

@@ -4,7 +4,9 @@
 // Counterpart of the reference's heap/heaputil.cuh:
 //   batchCopy   (heap/heaputil.cuh:4-12)   -> g2l_batch / l2g_batch / take_batch
 //   ibitonicSort(heap/heaputil.cuh:23-47)  -> wg_sort_batch
-//   imergePath  (heap/heaputil.cuh:99-158) -> wg_merge_path / wg_merge_bitonic
+//   imergePath  (heap/heaputil.cuh:99-158) -> wg_merge (one of wg_merge_bitonic,
+//                                             wg_merge_mp, wg_merge_grp; wg_merge_ser
+//                                             is a tested alternative)
 //
 // Layout conventions
 //   * A batch is B keys, ascending, B a power of two.
@@ -338,46 +340,16 @@ __device__ __forceinline__ void wg_sort_batch(K* dst, const K* src, int size, K 
 }
 
 // ---------------------------------------------------------------- merge path
-// Merge the sorted LDS batches a[B] and b[B]; thread t returns outputs
-// [t*L, t*L+L) of the merged 2B keys, L = 2B/NT.  Ties take `a` first, as in
-// imergePath (heap/heaputil.cuh:122-145).  Each thread finds its diagonal
-// split with a binary search over LDS, then merges serially with one LDS read
-// per output.
-template <typename K, int B, int NT>
-__device__ __forceinline__ void wg_merge_path(const K* a, const K* b, K (&out)[2 * B / NT]) {
-    constexpr int L = 2 * B / NT;
-    const int d = threadIdx.x * L;
-    int lo = d > B ? d - B : 0;
-    int hi = d < B ? d : B;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (!(b[d - mid - 1] < a[mid])) lo = mid + 1;  // a[mid] <= b[d-mid-1]
-        else hi = mid;
-    }
-    int ai = lo, bi = d - lo;
-    K av = a[ai < B ? ai : B - 1];
-    K bv = b[bi < B ? bi : B - 1];
-#pragma unroll
-    for (int s = 0; s < L; ++s) {
-        const bool ta = (bi >= B) || (ai < B && !(bv < av));
-        out[s] = ta ? av : bv;
-        if (ta) {
-            ++ai;
-            av = a[ai < B ? ai : B - 1];
-        } else {
-            ++bi;
-            bv = b[bi < B ? bi : B - 1];
-        }
-    }
-}
-
+// The merges below all produce the sorted union of a[B] and b[B] split at
+// B.  imergePath (heap/heaputil.cuh:122-145) takes `a` first on ties; for
+// scalar keys the tie rule leaves the output values unchanged.
+//
 // Merge path with a short dependent chain.  The diagonal split is found
 // with a 4-ary search (three independent LDS probes per round, so about
 // log4(B) rounds instead of log2(B)), then the thread loads the L keys that
 // follow the split in each input at once and keeps the L smallest of those
 // 2L keys: c[s] = min(x[s], y[L-1-s]) is bitonic and holds them, and
-// log2(L) in-register half-cleaning stages sort it.  Same outputs as
-// wg_merge_path (values are unaffected by the tie rule).
+// log2(L) in-register half-cleaning stages sort it.
 template <typename K>
 __device__ __forceinline__ K key_max() { return ~K(0); }
 
