@@ -602,9 +602,9 @@ __device__ __forceinline__ void wg_merge_bitonic(K* a, K* b, const DS& small, co
 // dependent chain is the search plus one LDS round trip.
 constexpr int kMergeGroup = 16;
 
-template <typename K, int B, int NT>
+template <typename K, int B, int NT, int GS = kMergeGroup>
 struct GroupMerge {
-    static constexpr int G = kMergeGroup;
+    static constexpr int G = GS;
     static constexpr int L = 2 * B / NT;
     static constexpr int W = G * L;
     static_assert(NT % G == 0 && B % W == 0, "group merge shape");
@@ -694,9 +694,9 @@ __device__ __forceinline__ void put_strided(const D& dst, int off, int j, const 
 // Merge a and b (LDS) with the group merge and write the B smallest keys to
 // `small`, the B largest to `large`.  Destinations may alias a or b.  Ends
 // with a barrier.
-template <typename K, int B, int NT, typename DS, typename DL>
+template <typename K, int B, int NT, typename DS, typename DL, int GS = kMergeGroup>
 __device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& small, const DL& large) {
-    using GM = GroupMerge<K, B, NT>;
+    using GM = GroupMerge<K, B, NT, GS>;
     K out[GM::L];
     int base, j;
     GM::run(a, b, out, base, j);
