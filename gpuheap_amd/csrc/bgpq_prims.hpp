@@ -99,6 +99,20 @@ constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 template <typename K> __device__ __forceinline__ K kmin(K a, K b) { return b < a ? b : a; }
 template <typename K> __device__ __forceinline__ K kmax(K a, K b) { return b < a ? a : b; }
 
+// One side of a compare-exchange whose side is a per-lane predicate:
+// take_min ? min(a, b) : max(a, b).  For 32-bit keys this is
+// med3(a, b, take_min ? 0 : ~0), which the compiler emits as a single
+// v_med3_u32 (the networks here are VALU-bound, and the select form costs
+// three operations); 64-bit keys keep the select form.
+template <typename K> __device__ __forceinline__ K kminmax(K a, K b, bool take_min) {
+    if constexpr (sizeof(K) == 4) {
+        const K k = take_min ? K(0) : ~K(0);
+        return kmax(kmin(a, b), kmin(kmax(a, b), k));
+    } else {
+        return take_min ? kmin(a, b) : kmax(a, b);
+    }
+}
+
 // xor-lane exchange inside a wave64.  Distances 1 and 2 use DPP quad_perm,
 // 4..16 ds_swizzle in bit mode (and=0x1f, xor=m), 32 ds_bpermute.  With
 // 1024-thread workgroups the sort and merge networks are VALU-bound (a
@@ -302,7 +316,7 @@ __device__ __forceinline__ void wg_sort_regs(K (&v)[P], K* xch) {
                             case 16: w = xlane<16>(v[i]); break;
                             default: w = xlane<32>(v[i]); break;
                         }
-                        v[i] = take_min ? kmin(v[i], w) : kmax(v[i], w);
+                        v[i] = kminmax(v[i], w, take_min);
                     }
                 } else {
                     store_run<K, P>(xch + t * P, v);
@@ -311,7 +325,7 @@ __device__ __forceinline__ void wg_sort_regs(K (&v)[P], K* xch) {
                     load_run<K, P>(w, xch + (t ^ m) * P);
                     __syncthreads();
 #pragma unroll
-                    for (int i = 0; i < P; ++i) v[i] = take_min ? kmin(v[i], w[i]) : kmax(v[i], w[i]);
+                    for (int i = 0; i < P; ++i) v[i] = kminmax(v[i], w[i], take_min);
                 }
             }
         }
@@ -526,8 +540,8 @@ __device__ __forceinline__ void xlane_minmax(K (&c)[P], K (&d)[P], int m, bool l
             case 16: wc = xlane<16>(c[i]); wd = xlane<16>(d[i]); break;
             default: wc = xlane<32>(c[i]); wd = xlane<32>(d[i]); break;
         }
-        c[i] = lower ? kmin(c[i], wc) : kmax(c[i], wc);
-        d[i] = lower ? kmin(d[i], wd) : kmax(d[i], wd);
+        c[i] = kminmax(c[i], wc, lower);
+        d[i] = kminmax(d[i], wd, lower);
     }
 }
 
@@ -574,8 +588,8 @@ __device__ __forceinline__ void wg_merge_bitonic(K* a, K* b, const DS& small, co
                 __syncthreads();
 #pragma unroll
                 for (int i = 0; i < P; ++i) {
-                    c[i] = lower ? kmin(c[i], wc[i]) : kmax(c[i], wc[i]);
-                    d[i] = lower ? kmin(d[i], wd[i]) : kmax(d[i], wd[i]);
+                    c[i] = kminmax(c[i], wc[i], lower);
+                    d[i] = kminmax(d[i], wd[i], lower);
                 }
             }
         }
@@ -671,7 +685,7 @@ struct GroupMerge {
 #pragma unroll
             for (int s = 0; s < L; ++s) {
                 const K w = xlane<M>(out[s]);
-                out[s] = lower ? kmin(out[s], w) : kmax(out[s], w);
+                out[s] = kminmax(out[s], w, lower);
             }
             xclean<M / 2>(out, j);
         }
@@ -707,14 +721,15 @@ __device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& s
 }
 
 // The merge used by the heap, by measurement on MI355X (one workgroup,
-// LDS inputs): the register bitonic merge below B = 1024 (B = 512: 0.63 us
-// against 0.84 for the merge path), the merge path with a 4-ary split
-// search up to B = 4096 (B = 2048: 1.2 us against 1.9 bitonic and 1.6
-// group), and the group merge at B = 8192 (4.2 us against 5.7).
+// LDS inputs, u32 keys): the register bitonic merge up to B = 1024
+// (B = 1024, 512 threads: 0.80 us against 0.93 for the merge path), the
+// merge path with a 4-ary split search for B = 2048 and 4096 (B = 2048:
+// 1.18 us against 1.47 bitonic and 1.48 group), and the group merge at
+// B = 8192 (3.4 us against 4.9 merge path and 5.6 bitonic).
 template <typename K, int B, int NT, typename DS, typename DL>
 __device__ __forceinline__ void wg_merge(K* a, K* b, const DS& small, const DL& large) {
     if constexpr (B >= 8192) wg_merge_grp<K, B, NT>(a, b, small, large);
-    else if constexpr (B >= 1024) wg_merge_mp<K, B, NT>(a, b, small, large);
+    else if constexpr (B >= 2048) wg_merge_mp<K, B, NT>(a, b, small, large);
     else wg_merge_bitonic<K, B, NT>(a, b, small, large);
 }
 

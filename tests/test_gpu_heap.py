@@ -227,3 +227,16 @@ def test_output_capacity_guard(gpu):
     G.sync()
     assert h.isEmpty()
     assert np.array_equal(rest.to_numpy(np.uint32, n - 3 * B), srt[3 * B:])
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("B", [64, 128, 256])
+def test_concurrent_stress(gpu, variant, B):
+    """Small batches and many workgroups maximise lock hand-offs: the
+    concurrent harness on several seeds must still emit the sorted keys and
+    leave the heap valid after the insertion phase."""
+    for seed in range(4):
+        print(f"[case] stress {variant} B={B} seed={seed}", flush=True)
+        got, want, (ok, msg) = _harness(variant, B, 1 << 16, "u32", seed=100 + seed, grid=128)
+        assert ok, msg
+        assert np.array_equal(got, want), f"seed {seed}"
