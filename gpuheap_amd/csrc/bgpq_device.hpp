@@ -90,6 +90,13 @@ __device__ __forceinline__ int* bcast_slots() {
     return slots;
 }
 
+// LDS word through which one lane broadcasts a key.
+template <typename K>
+__device__ __forceinline__ K* key_slot() {
+    __shared__ K slot;
+    return &slot;
+}
+
 template <typename K, int B>
 constexpr int insert_lds_bytes() { return 2 * B * (int)sizeof(K); }
 template <typename K, int B>
@@ -416,63 +423,113 @@ struct DevHeap {
         sift_down(X0, X1, X2);
     }
 
-    // ---- sift the candidate X0 down from the root, which the caller holds
+    // bu: lanes 0 and 1 request the locks of the two children of `nxt`
+    // without waiting; the state word found (INUSE if another workgroup holds
+    // it), or kNoPre.  The next level's lock phase takes the result.
+    static constexpr int kNoPre = INT_MIN;
+    __device__ __forceinline__ int lock_ahead(int nxt) const {
+        if (threadIdx.x < 2) {
+            const int c = threadIdx.x == 0 ? left_child(nxt) : right_child(nxt);
+            if (c <= h.batch_num) return a_max(st(c), INUSE);
+        }
+        return kNoPre;
+    }
+
+    // ---- sift the candidate X0 down from the root, which the caller holds.
+    // Each level merges the two children: the larger half goes back to the
+    // child with the larger maximum (tgt), the smaller half competes with the
+    // candidate (buitdd:422-443), and the sift continues at the other child
+    // (oth).  The next deleter waits for the node this level releases, so for
+    // B >= 4096 the node's hold is kept short:
+    //  - the smaller half is computed first; when the candidate is no smaller
+    //    than all of it (the usual case) it is the node's new batch, and the
+    //    node is published and released before the larger half is computed;
+    //  - the larger child's new batch is then stored without a publish of its
+    //    own and released with the next level's publish ("pending"): no other
+    //    workgroup can want it before the node below is released, which
+    //    happens at that same publish;
+    //  - bu: the locks of oth's children are requested as soon as the descent
+    //    is known (and given back if the sift stops here), and when both are
+    //    granted their batches are loaded while the larger half is computed,
+    //    so the next level starts with its children in LDS.
+    // Every wait is for a node deeper than every node held, as before.
     __device__ void sift_down(K* X0, K* X1, K* X2) const {
+        constexpr bool kSplit = B >= 4096;
+        constexpr bool kAhead = BU && kSplit;
         int* bc = bcast_slots();
         int cur = 1;
         int cur_prev = AVAIL;
+        int pend = 0, pend_word = AVAIL;  // stored, released with the next publish
+        int pre = kNoPre;                 // lanes 0, 1: result of lock_ahead
+        bool ready = false;               // X1, X2 hold cur's children, already locked
+        int ready_lp = AVAIL, ready_rp = AVAIL;
         while (true) {
             const int l = left_child(cur), r = right_child(cur);
             const bool lin = l <= h.batch_num, rin = r <= h.batch_num;
             BGPQ_EVENT(10);
-            if constexpr (BU) {
-                // lanes 0 and 1 lock the two children at the same time
-                if (threadIdx.x < 2) {
-                    const int c = threadIdx.x == 0 ? l : r;
-                    const bool in = threadIdx.x == 0 ? lin : rin;
-                    int p = in ? lock_max(c) : AVAIL;
-                    if (is_inshold(p)) p = DELMOD;
-                    bc[BC_C + threadIdx.x] = p;
-                }
-                if (threadIdx.x == 0) bc[BC_E] = 2;
+            int lp, rp, type;
+            if (ready) {
+                ready = false;
+                lp = ready_lp;
+                rp = ready_rp;
+                type = 2;
             } else {
-                if (threadIdx.x == 0) {
-                    int type = 2;
-                    // a child that is an insert target in flight is treated as
-                    // absent (tditdd:425-446)
-                    if (lin) {
-                        const int s = lock_child_td(l);
-                        if (s != AVAIL) type = 0;
+                if constexpr (BU) {
+                    // lanes 0 and 1 lock the two children at the same time
+                    if (threadIdx.x < 2) {
+                        const int c = threadIdx.x == 0 ? l : r;
+                        const bool in = threadIdx.x == 0 ? lin : rin;
+                        int p = AVAIL;
+                        if (in) p = (pre != kNoPre && pre != INUSE) ? pre : lock_max(c);
+                        if (is_inshold(p)) p = DELMOD;
+                        bc[BC_C + threadIdx.x] = p;
                     }
-                    if (type == 2 && rin) {
-                        const int s = lock_child_td(r);
-                        if (s != AVAIL) type = 1;
+                    pre = kNoPre;
+                    if (threadIdx.x == 0) bc[BC_E] = 2;
+                } else {
+                    if (threadIdx.x == 0) {
+                        int t2 = 2;
+                        // a child that is an insert target in flight is treated
+                        // as absent (tditdd:425-446)
+                        if (lin) {
+                            const int s = lock_child_td(l);
+                            if (s != AVAIL) t2 = 0;
+                        }
+                        if (t2 == 2 && rin) {
+                            const int s = lock_child_td(r);
+                            if (s != AVAIL) t2 = 1;
+                        }
+                        bc[BC_C] = AVAIL;
+                        bc[BC_D] = AVAIL;
+                        bc[BC_E] = t2;
                     }
-                    bc[BC_C] = AVAIL;
-                    bc[BC_D] = AVAIL;
-                    bc[BC_E] = type;
                 }
+                __syncthreads();
+                BGPQ_EVENT(11);
+                if (aborted()) return;
+                lp = bc[BC_C];
+                rp = bc[BC_D];
+                type = bc[BC_E];
+                if (type != 0) {
+                    if (lin) g2l_batch<K, B, NT>(X1, node(l));
+                    else fill_lds<K, B, NT>(X1, h.init_limits);
+                }
+                if (type == 2) {
+                    if (rin) g2l_batch<K, B, NT>(X2, node(r));
+                    else fill_lds<K, B, NT>(X2, h.init_limits);
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            BGPQ_EVENT(11);
-            if (aborted()) return;
-            const int lp = bc[BC_C], rp = bc[BC_D], type = bc[BC_E];
-            if (type != 0) {
-                if (lin) g2l_batch<K, B, NT>(X1, node(l));
-                else fill_lds<K, B, NT>(X1, h.init_limits);
-            }
-            if (type == 2) {
-                if (rin) g2l_batch<K, B, NT>(X2, node(r));
-                else fill_lds<K, B, NT>(X2, h.init_limits);
-            }
-            __syncthreads();
             BGPQ_EVENT(12);
 
             if constexpr (!BU) {
                 if (type == 0) {  // no child: the candidate stays here
                     l2g_batch<K, B, NT>(node(cur), X0);
                     publish();
-                    if (threadIdx.x == 0) release_st(cur, AVAIL);
+                    if (threadIdx.x == 0) {
+                        release_st(cur, AVAIL);
+                        if (pend) release_st(pend, pend_word);
+                    }
                     return;
                 }
                 if (type == 1) {  // only the left child, which is a leaf
@@ -481,28 +538,94 @@ struct DevHeap {
                     if (threadIdx.x == 0) {
                         release_st(cur, AVAIL);
                         release_st(l, AVAIL);
+                        if (pend) release_st(pend, pend_word);
                     }
                     return;
                 }
             }
 
-            // merge the two children: the larger half goes back to the child
-            // with the larger maximum, the smaller half (X1) competes with the
-            // candidate (buitdd:422-443)
             const bool tr = X1[B - 1] < X2[B - 1];
             const int tgt = tr ? r : l, oth = tr ? l : r;
             const bool tgt_in = tr ? rin : lin, oth_in = tr ? lin : rin;
             const int tgt_prev = tr ? rp : lp, oth_prev = tr ? lp : rp;
-            if (tgt_in) wg_merge<K, B, NT>(X1, X2, ToLds<K>{X1}, ToGlobal<K>{node(tgt)});
-            else wg_merge<K, B, NT>(X1, X2, ToLds<K>{X1}, ToNone{});
-            BGPQ_EVENT(13);
-
-            const K cmin = X0[0], cmax = X0[B - 1], kmin_ = X1[0], kmax_ = X1[B - 1];
+            const K cmin = X0[0], cmax = X0[B - 1];
+            const K kmin_ = kmin(X1[0], X2[0]);  // the children's smallest key
             const bool stop = (VAR == VAR_BUITDD) ? (cmax < kmin_) : !(kmin_ < cmax);
-            if (!(cmin < kmax_)) {
+            if constexpr (kSplit) {
+                if constexpr (kAhead) {
+                    if (oth_in) pre = lock_ahead(oth);
+                }
+                HalfMerge<K, B, NT, false> lo;
+                lo.run(X1, X2);
+                if (threadIdx.x == NT - 1) *key_slot<K>() = lo.last();
+                if constexpr (kAhead) {
+                    if (threadIdx.x < 2) bc[BC_C + threadIdx.x] = pre;
+                }
+                __syncthreads();
+                BGPQ_EVENT(13);
+                if (!(cmin < *key_slot<K>()) && oth_in) {
+                    // candidate >= all of the small half: the node takes it
+                    lo.put(ToGlobal<K>{node(cur)});
+                    publish();
+                    if (threadIdx.x == 0) {
+                        release_st(cur, BU ? cur_prev : AVAIL);
+                        if (pend) release_st(pend, pend_word);
+                    }
+                    BGPQ_EVENT(17);
+                    bool fetch = false;
+                    int plp = AVAIL, prp = AVAIL;
+                    BatchLoad<K, B, NT> fl, fr;
+                    if constexpr (kAhead) {
+                        const int nl = left_child(oth), nr = right_child(oth);
+                        plp = bc[BC_C];
+                        prp = bc[BC_D];
+                        fetch = nl <= h.batch_num && nr <= h.batch_num && plp != kNoPre && plp != INUSE &&
+                                prp != kNoPre && prp != INUSE;
+                        if (fetch) {
+                            fl.issue(node(nl));
+                            fr.issue(node(nr));
+                        }
+                    }
+                    HalfMerge<K, B, NT, true> hi;
+                    hi.run(X1, X2);
+                    if (tgt_in) hi.put(ToGlobal<K>{node(tgt)});
+                    pend = tgt_in ? tgt : 0;
+                    pend_word = BU ? tgt_prev : AVAIL;
+                    __syncthreads();  // X1 and X2 are read
+                    if (fetch) {
+                        fl.commit(X1);
+                        fr.commit(X2);
+                        __syncthreads();
+                        ready = true;
+                        ready_lp = is_inshold(plp) ? DELMOD : plp;
+                        ready_rp = is_inshold(prp) ? DELMOD : prp;
+                        pre = kNoPre;
+                    }
+                    BGPQ_EVENT(15);
+                    cur = oth;
+                    cur_prev = oth_prev;
+                    continue;
+                }
+                // the candidate takes part: both halves, the small one into X1
+                HalfMerge<K, B, NT, true> hi;
+                hi.run(X1, X2);
+                __syncthreads();  // every read of X1 and X2 is done
+                lo.put(ToLds<K>{X1});
+                if (tgt_in) hi.put(ToGlobal<K>{node(tgt)});
+                __syncthreads();
+            } else {
+                if (tgt_in) wg_merge<K, B, NT>(X1, X2, ToLds<K>{X1}, ToGlobal<K>{node(tgt)});
+                else wg_merge<K, B, NT>(X1, X2, ToLds<K>{X1}, ToNone{});
+                BGPQ_EVENT(13);
+            }
+
+            if (!(cmin < X1[B - 1]) && oth_in) {
                 // candidate >= all of the children's small half: the node takes it
                 l2g_batch<K, B, NT>(node(cur), X1);
-            } else if (stop) {
+            } else if (stop || !oth_in) {
+                // the candidate stays here and the small half goes to oth
+                // (without oth, keys equal to init_limits were inserted,
+                // which is outside the contract)
                 l2g_batch<K, B, NT>(node(cur), X0);
                 if (oth_in) l2g_batch<K, B, NT>(node(oth), X1);
                 publish();
@@ -510,9 +633,27 @@ struct DevHeap {
                     if (tgt_in) release_st(tgt, BU ? tgt_prev : AVAIL);
                     release_st(cur, BU ? cur_prev : AVAIL);
                     if (oth_in) release_st(oth, BU ? oth_prev : AVAIL);
+                    if (pend) release_st(pend, pend_word);
                 }
+                if constexpr (kAhead) {
+                    // give back the locks requested ahead, untouched
+                    if (threadIdx.x < 2 && pre != kNoPre && pre != INUSE)
+                        a_store(st(threadIdx.x == 0 ? left_child(oth) : right_child(oth)), pre);
+                }
+                if (!stop) flag_error(ERR_LOST_KEYS);
                 BGPQ_EVENT(16);
                 return;
+            } else if constexpr (kSplit) {
+                // the node takes the B smallest of candidate and small half,
+                // the candidate keeps the rest (two halves: fewer registers)
+                HalfMerge<K, B, NT, false> lo2;
+                lo2.run(X0, X1);
+                lo2.put(ToGlobal<K>{node(cur)});
+                HalfMerge<K, B, NT, true> hi2;
+                hi2.run(X0, X1);
+                __syncthreads();  // every read of X0 is done
+                hi2.put(ToLds<K>{X0});
+                __syncthreads();
             } else {
                 wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(cur)}, ToLds<K>{X0});
             }
@@ -521,13 +662,10 @@ struct DevHeap {
             if (threadIdx.x == 0) {
                 if (tgt_in) release_st(tgt, BU ? tgt_prev : AVAIL);
                 release_st(cur, BU ? cur_prev : AVAIL);
+                if (pend) release_st(pend, pend_word);
             }
+            pend = 0;
             BGPQ_EVENT(15);
-            if (!oth_in) {
-                // only reachable when keys equal init_limits (outside the contract)
-                flag_error(ERR_LOST_KEYS);
-                return;
-            }
             cur = oth;
             cur_prev = oth_prev;
         }
@@ -681,19 +819,41 @@ struct DevHeap {
                 BGPQ_EVENT(26);
                 const bool identity = !(X0[0] < X1[B - 1]);  // parent.max <= node.min
                 const bool done = identity && !dirty;
-                if (!identity) {
+                const int par_word = (done || par == 1) ? AVAIL : mine;
+                if (identity) {
+                    publish();
+                    if (threadIdx.x == 0) {
+                        release_st(par, par_word);
+                        release_st(cur, AVAIL);
+                    }
+                } else {
+                    // The parent's new batch is stored and the parent released
+                    // first (every insertion that reaches the root waits for
+                    // it); the node stays locked until it holds the larger
+                    // half, so no one reads it in between.
                     if (!(X1[0] < X0[B - 1])) {
                         // parent.min >= node.max: swap (buitdd:647-661)
                         l2g_batch<K, B, NT>(node(par), X0);
+                        publish();
+                        if (threadIdx.x == 0) release_st(par, par_word);
                         l2g_batch<K, B, NT>(node(cur), X1);
-                    } else {
+                    } else if constexpr (B < 4096) {
+                        // small batches: one full merge is cheaper than two halves
                         wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(par)}, ToGlobal<K>{node(cur)});
+                        publish();
+                        if (threadIdx.x == 0) release_st(par, par_word);
+                    } else {
+                        HalfMerge<K, B, NT, false> lo;
+                        lo.run(X0, X1);
+                        lo.put(ToGlobal<K>{node(par)});
+                        publish();
+                        if (threadIdx.x == 0) release_st(par, par_word);
+                        HalfMerge<K, B, NT, true> hi;
+                        hi.run(X0, X1);
+                        hi.put(ToGlobal<K>{node(cur)});
                     }
-                }
-                publish();
-                if (threadIdx.x == 0) {
-                    release_st(par, (done || par == 1) ? AVAIL : mine);
-                    release_st(cur, AVAIL);
+                    publish();
+                    if (threadIdx.x == 0) release_st(cur, AVAIL);
                 }
                 BGPQ_EVENT(done ? 28 : 27);
                 if (done) return;

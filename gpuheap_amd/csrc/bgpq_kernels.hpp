@@ -93,7 +93,8 @@ __global__ __launch_bounds__(threads_for(B)) void k_prim_sort(const K* in, const
     l2g_batch<K, B, NT>(out + (size_t)blockIdx.x * B, X0);
 }
 
-// algo 0: the register bitonic merge, 1: the LDS merge path
+// algo 0: the register bitonic merge, 1: the LDS merge path, 2: the group
+// merge, 3: the serial merge path, 4: the two halves one after the other
 template <typename K, int B, int ALGO>
 __global__ __launch_bounds__(threads_for(B)) void k_prim_merge(const K* a, const K* b, K* small, K* large) {
     constexpr int NT = threads_for(B);
@@ -107,7 +108,16 @@ __global__ __launch_bounds__(threads_for(B)) void k_prim_merge(const K* a, const
     if constexpr (ALGO == 0) wg_merge_bitonic<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
     else if constexpr (ALGO == 1) wg_merge_mp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
     else if constexpr (ALGO == 2) wg_merge_grp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
-    else wg_merge_ser<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
+    else if constexpr (ALGO == 3) wg_merge_ser<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
+    else {
+        // the two halves one after the other, as the heap's sift steps take them
+        HalfMerge<K, B, NT, false> lo;
+        lo.run(X0, X1);
+        lo.put(ToGlobal<K>{small + o});
+        HalfMerge<K, B, NT, true> hi;
+        hi.run(X0, X1);
+        hi.put(ToGlobal<K>{large + o});
+    }
 }
 
 template <typename K>
@@ -189,34 +199,25 @@ struct Launch {
                            sizes, (K*)out, (K)pad);
         return hipGetLastError();
     }
+    template <int ALGO>
+    static hipError_t merge_as(const void* a, const void* b, void* small, void* large, int batches, hipStream_t s) {
+        static const hipError_t ok =
+            allow_lds(reinterpret_cast<const void*>(&k_prim_merge<K, B, ALGO>), 2 * B * sizeof(K));
+        if (ok != hipSuccess) return ok;
+        clear_last_error();
+        hipLaunchKernelGGL((k_prim_merge<K, B, ALGO>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
+                           (const K*)a, (const K*)b, (K*)small, (K*)large);
+        return hipGetLastError();
+    }
     static hipError_t merge(const void* a, const void* b, void* small, void* large, int batches, int algo,
                             hipStream_t s) {
-        static const hipError_t ok0 =
-            allow_lds(reinterpret_cast<const void*>(&k_prim_merge<K, B, 0>), 2 * B * sizeof(K));
-        static const hipError_t ok1 =
-            allow_lds(reinterpret_cast<const void*>(&k_prim_merge<K, B, 1>), 2 * B * sizeof(K));
-        static const hipError_t ok2 =
-            allow_lds(reinterpret_cast<const void*>(&k_prim_merge<K, B, 2>), 2 * B * sizeof(K));
-        static const hipError_t ok3 =
-            allow_lds(reinterpret_cast<const void*>(&k_prim_merge<K, B, 3>), 2 * B * sizeof(K));
-        if (ok0 != hipSuccess) return ok0;
-        if (ok1 != hipSuccess) return ok1;
-        if (ok2 != hipSuccess) return ok2;
-        if (ok3 != hipSuccess) return ok3;
-        clear_last_error();
-        if (algo == 0)
-            hipLaunchKernelGGL((k_prim_merge<K, B, 0>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
-                               (const K*)a, (const K*)b, (K*)small, (K*)large);
-        else if (algo == 1)
-            hipLaunchKernelGGL((k_prim_merge<K, B, 1>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
-                               (const K*)a, (const K*)b, (K*)small, (K*)large);
-        else if (algo == 2)
-            hipLaunchKernelGGL((k_prim_merge<K, B, 2>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
-                               (const K*)a, (const K*)b, (K*)small, (K*)large);
-        else
-            hipLaunchKernelGGL((k_prim_merge<K, B, 3>), dim3(batches), dim3(threads_for(B)), 2 * B * sizeof(K), s,
-                               (const K*)a, (const K*)b, (K*)small, (K*)large);
-        return hipGetLastError();
+        switch (algo) {
+            case 0: return merge_as<0>(a, b, small, large, batches, s);
+            case 1: return merge_as<1>(a, b, small, large, batches, s);
+            case 2: return merge_as<2>(a, b, small, large, batches, s);
+            case 3: return merge_as<3>(a, b, small, large, batches, s);
+            default: return merge_as<4>(a, b, small, large, batches, s);
+        }
     }
     static LaunchRow row() { return LaunchRow{B, &sort, &merge, &insert, &del, &conc}; }
 };

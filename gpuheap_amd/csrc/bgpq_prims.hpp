@@ -20,6 +20,7 @@
 //     the others the large half, so every store is a contiguous L-key run.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <stdint.h>
 
 #include <type_traits>
@@ -152,6 +153,32 @@ __device__ __forceinline__ void g2l_batch(K* __restrict__ lds, const K* __restri
         if (NV % NT == 0 || c < NV) d[c] = gload16(r, c * 16);
     }
 }
+// g2l_batch in two steps: issue() starts the coherent loads into registers
+// and returns at once; commit() writes them to LDS (waiting for them there).
+// Other work placed between the two hides the load latency.
+template <typename K, int B, int NT>
+struct BatchLoad {
+    static constexpr int NV = B * (int)sizeof(K) / 16;
+    static constexpr int PER = (NV + NT - 1) / NT;
+    uint4 q[PER];
+    __device__ __forceinline__ void issue(const K* g) {
+        const __amdgpu_buffer_rsrc_t r = buf_rsrc(g);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int c = threadIdx.x + k * NT;
+            if (NV % NT == 0 || c < NV) q[k] = gload16(r, c * 16);
+        }
+    }
+    __device__ __forceinline__ void commit(K* lds) const {
+        uint4* d = reinterpret_cast<uint4*>(lds);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int c = threadIdx.x + k * NT;
+            if (NV % NT == 0 || c < NV) d[c] = q[k];
+        }
+    }
+};
+
 template <typename K, int B, int NT>
 __device__ __forceinline__ void l2g_batch(K* __restrict__ g, const K* __restrict__ lds) {
     constexpr int NV = B * (int)sizeof(K) / 16;
@@ -368,9 +395,11 @@ template <typename K>
 __device__ __forceinline__ K key_max() { return ~K(0); }
 
 template <typename K, int B, int NT>
-__device__ __forceinline__ void wg_merge_path_fast(const K* a, const K* b, K (&out)[2 * B / NT]) {
-    constexpr int L = 2 * B / NT;
-    const int d = threadIdx.x * L;
+__device__ __forceinline__ void wg_merge_path_fast(const K* a, const K* b, K (&out)[2 * B / NT]);
+
+// Outputs d .. d+L-1 of the merge of a[B] and b[B] (0 <= d, d+L <= 2B).
+template <typename K, int B, int L>
+__device__ __forceinline__ void merge_path_at(const K* a, const K* b, int d, K (&out)[L]) {
     int lo = d > B ? d - B : 0;
     int hi = d < B ? d : B;
     // f(i) = a[i] is among the first d outputs = !(b[d-1-i] < a[i]); true then false
@@ -423,6 +452,12 @@ __device__ __forceinline__ void wg_merge_path_fast(const K* a, const K* b, K (&o
             }
         }
     }
+}
+
+template <typename K, int B, int NT>
+__device__ __forceinline__ void wg_merge_path_fast(const K* a, const K* b, K (&out)[2 * B / NT]) {
+    constexpr int L = 2 * B / NT;
+    merge_path_at<K, B, L>(a, b, (int)threadIdx.x * L, out);
 }
 
 // Destination tags for the halves of a merge: LDS, global heap node
@@ -628,8 +663,15 @@ struct GroupMerge {
     __device__ __forceinline__ static void run(const K* a, const K* b, K (&out)[L], int& base, int& j) {
         const int t = threadIdx.x;
         j = t % G;
-        const int D = (t / G) * W;
-        base = D;
+        base = (t / G) * W;
+        window<L>(a, b, base, j, out);
+    }
+
+    // One group's window of G*LL outputs starting at output D; lane j gets
+    // outputs D + s*G + j (s < LL).  D is the same for the G lanes.
+    template <int LL>
+    __device__ __forceinline__ static void window(const K* a, const K* b, int D, int j, K (&out)[LL]) {
+        constexpr int W = G * LL;
         int lo = D > B ? D - B : 0;
         int hi = D < B ? D : B;
         const int shift = ((int)__lane_id() / G) * G;
@@ -648,15 +690,15 @@ struct GroupMerge {
         const int ai = lo, bi = D - lo;
         // unconditional loads at clamped indexes, then a select (a guarded
         // load would become a branch with its own wait per key)
-        K x[L], y[L];
+        K x[LL], y[LL];
 #pragma unroll
-        for (int s = 0; s < L; ++s) {
+        for (int s = 0; s < LL; ++s) {
             const int p = s * G + j;
             x[s] = a[kmin(ai + p, B - 1)];
             y[s] = b[kmin(bi + W - 1 - p, B - 1)];
         }
 #pragma unroll
-        for (int s = 0; s < L; ++s) {
+        for (int s = 0; s < LL; ++s) {
             const int p = s * G + j;
             const K xs = ai + p < B ? x[s] : key_max<K>();
             const K ys = bi + W - 1 - p < B ? y[s] : key_max<K>();
@@ -664,9 +706,9 @@ struct GroupMerge {
         }
         // strides W/2 .. G: partner position p ^ stride is column j, row s ^ (stride/G)
 #pragma unroll
-        for (int r = L / 2; r > 0; r >>= 1) {
+        for (int r = LL / 2; r > 0; r >>= 1) {
 #pragma unroll
-            for (int s = 0; s < L; ++s) {
+            for (int s = 0; s < LL; ++s) {
                 if ((s & r) == 0) {
                     const K u = out[s], v = out[s | r];
                     out[s] = kmin(u, v);
@@ -675,23 +717,62 @@ struct GroupMerge {
             }
         }
         // strides G/2 .. 1: partner lane j ^ m
-        xclean<G / 2>(out, j);
+        xclean<G / 2, LL>(out, j);
     }
 
-    template <int M>
-    __device__ __forceinline__ static void xclean(K (&out)[L], int j) {
+    template <int M, int LL>
+    __device__ __forceinline__ static void xclean(K (&out)[LL], int j) {
         if constexpr (M >= 1) {
             const bool lower = (j & M) == 0;
 #pragma unroll
-            for (int s = 0; s < L; ++s) {
+            for (int s = 0; s < LL; ++s) {
                 const K w = xlane<M>(out[s]);
                 out[s] = kminmax(out[s], w, lower);
             }
-            xclean<M / 2>(out, j);
+            xclean<M / 2, LL>(out, j);
         }
     }
 };
 
+// 4x4 transpose inside each quad of lanes: lane r of a quad holds
+// M[r][c] = v[4m+c] (for each block m of 4 registers) and ends up with
+// M[c][r], c = 0..3.  Two butterfly steps, each a DPP quad_perm exchange and
+// a select per register.
+template <int X>
+__device__ __forceinline__ uint32_t quad_xor(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, X == 1 ? 0xB1 : 0x4E, 0xF, 0xF, false);
+}
+template <int X>
+__device__ __forceinline__ uint64_t quad_xor(uint64_t v) {
+    return ((uint64_t)quad_xor<X>((uint32_t)(v >> 32)) << 32) | quad_xor<X>((uint32_t)v);
+}
+template <typename K, int L>
+__device__ __forceinline__ void quad_transpose(K (&v)[L]) {
+    static_assert(L % 4 == 0, "quad transpose");
+    const int r = (int)__lane_id() & 3;
+#pragma unroll
+    for (int m = 0; m < L; m += 4) {
+        // step 1: 2x2 blocks, rows/columns differing in bit 0
+#pragma unroll
+        for (int c = 0; c < 4; c += 2) {
+            const K t0 = quad_xor<1>(v[m + c + 1]), t1 = quad_xor<1>(v[m + c]);
+            v[m + c] = (r & 1) ? t0 : v[m + c];
+            v[m + c + 1] = (r & 1) ? v[m + c + 1] : t1;
+        }
+        // step 2: rows/columns differing in bit 1
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const K t0 = quad_xor<2>(v[m + c + 2]), t1 = quad_xor<2>(v[m + c]);
+            v[m + c] = (r & 2) ? t0 : v[m + c];
+            v[m + c + 2] = (r & 2) ? v[m + c + 2] : t1;
+        }
+    }
+}
+
+// Store the group layout (lane j of a G-lane group holds outputs off + s*G + j).
+// To global memory with L % 4 == 0, a quad transpose first gives lane
+// 4q + r the four consecutive keys off + (4m + r)*G + 4q .. +3 of each block
+// m, which go out as 16-byte stores (u32; two for u64) instead of 4-byte ones.
 template <typename K, int L, int G, typename D>
 __device__ __forceinline__ void put_strided(const D& dst, int off, int j, const K (&v)[L]) {
     if constexpr (std::is_same<D, ToNone>::value) {
@@ -699,6 +780,26 @@ __device__ __forceinline__ void put_strided(const D& dst, int off, int j, const 
     } else if constexpr (std::is_same<D, ToLds<K>>::value) {
 #pragma unroll
         for (int s = 0; s < L; ++s) dst.p[off + s * G + j] = v[s];
+    } else if constexpr (L % 4 == 0 && G % 4 == 0) {
+        K w[L];
+#pragma unroll
+        for (int s = 0; s < L; ++s) w[s] = v[s];
+        quad_transpose<K, L>(w);
+        const int r = j & 3, q4 = j & ~3;
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(dst.p);
+#pragma unroll
+        for (int m = 0; m < L; m += 4) {
+            const int e = off + (m + r) * G + q4;  // first key of this lane's run
+            if constexpr (sizeof(K) == 4) {
+                gstore16(rs, e * 4, make_uint4((uint32_t)w[m], (uint32_t)w[m + 1], (uint32_t)w[m + 2],
+                                              (uint32_t)w[m + 3]));
+            } else {
+                gstore16(rs, e * 8, make_uint4((uint32_t)w[m], (uint32_t)((uint64_t)w[m] >> 32),
+                                              (uint32_t)w[m + 1], (uint32_t)((uint64_t)w[m + 1] >> 32)));
+                gstore16(rs, e * 8 + 16, make_uint4((uint32_t)w[m + 2], (uint32_t)((uint64_t)w[m + 2] >> 32),
+                                                   (uint32_t)w[m + 3], (uint32_t)((uint64_t)w[m + 3] >> 32)));
+            }
+        }
     } else {
 #pragma unroll
         for (int s = 0; s < L; ++s) gstore_key(dst.p + off + s * G + j, v[s]);
@@ -719,6 +820,39 @@ __device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& s
     else put_strided<K, GM::L, GM::G>(large, base - B, j, out);
     __syncthreads();
 }
+
+// One half of a merge: the B smallest (HI = false) or the B largest
+// (HI = true) keys of the union of sorted a[B] and b[B], sorted, N = B/NT of
+// them in each thread's registers.  a and b are only read, so both halves
+// can be taken from the same inputs one after the other, and the first can
+// be stored (and its node released) before the second is computed.  Thread
+// 0's v[0] is the half's smallest key and thread NT-1's v[N-1] its largest.
+// B = 8192 uses the group merge's windows, smaller B the merge path.
+template <typename K, int B, int NT, bool HI>
+struct HalfMerge {
+    static constexpr int N = B / NT;
+    static constexpr bool kGroup = B >= 8192;
+    static constexpr int G = kMergeGroup;
+    static_assert(!kGroup || NT % G == 0, "half merge shape");
+    K v[N];
+
+    __device__ __forceinline__ void run(const K* a, const K* b) {
+        const int t = threadIdx.x;
+        if constexpr (kGroup) {
+            GroupMerge<K, B, NT>::template window<N>(a, b, (HI ? B : 0) + (t / G) * (G * N), t % G, v);
+        } else {
+            merge_path_at<K, B, N>(a, b, (HI ? B : 0) + t * N, v);
+        }
+    }
+    // Store the half at positions [0, B) of dst.
+    template <typename D>
+    __device__ __forceinline__ void put(const D& dst) const {
+        const int t = threadIdx.x;
+        if constexpr (kGroup) put_strided<K, N, G>(dst, (t / G) * (G * N), t % G, v);
+        else put_run<K, N>(dst, t * N, v);
+    }
+    __device__ __forceinline__ K last() const { return v[N - 1]; }
+};
 
 // The merge used by the heap, by measurement on MI355X (one workgroup,
 // LDS inputs, u32 keys): the register bitonic merge up to B = 1024

@@ -1,8 +1,9 @@
 """The batch primitives on the GPU against numpy, for every built batch size:
 the insert path's pad-and-sort (ibitonicSort, heap/heaputil.cuh:23-47) must
-equal numpy's sort of the padded batch, and both merge kernels (imergePath,
-heap/heaputil.cuh:99-158: the register bitonic merge the heap uses and the
-LDS merge path) must split the sorted union at B."""
+equal numpy's sort of the padded batch, and every merge kernel (imergePath,
+heap/heaputil.cuh:99-158: register bitonic, merge path, group merge, serial
+merge path, and the two halves taken one after the other as the heap's sift
+steps take them) must split the sorted union at B."""
 import numpy as np
 import pytest
 
@@ -47,7 +48,7 @@ def test_sort_batch(gpu, B, key):
 
 @pytest.mark.parametrize("B", BATCHES)
 @pytest.mark.parametrize("key", ["u32", "u64"])
-@pytest.mark.parametrize("algo", [0, 1, 2, 3], ids=["bitonic", "merge_path", "group", "serial"])
+@pytest.mark.parametrize("algo", [0, 1, 2, 3, 4], ids=["bitonic", "merge_path", "group", "serial", "halves"])
 def test_merge_batches(gpu, B, key, algo):
     kt, dtype = KEYS[key]
     if key == "u64" and B == 8192:
