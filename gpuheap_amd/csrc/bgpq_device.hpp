@@ -175,6 +175,18 @@ struct DevHeap {
             backoff();
         }
     }
+    // lock_max for a deleter's children, where at most a workgroup or two
+    // wait for a node: the waiter retries the atomic itself, so the attempt
+    // that follows the holder's release takes the lock one round trip after
+    // it (polling first would cost two).
+    __device__ __forceinline__ int lock_max_eager(int i) const {
+        int n = 0;
+        while (true) {
+            const int p = a_max(st(i), INUSE);
+            if (p != INUSE) return p;
+            if (poll_abort(n)) return -1;
+        }
+    }
     // td child lock: AVAIL -> INUSE, waiting while INUSE; returns the state
     // found (AVAIL means locked), or -1 on abort
     __device__ __forceinline__ int lock_child_td(int i) const {
@@ -480,7 +492,7 @@ struct DevHeap {
                         const int c = threadIdx.x == 0 ? l : r;
                         const bool in = threadIdx.x == 0 ? lin : rin;
                         int p = AVAIL;
-                        if (in) p = (pre != kNoPre && pre != INUSE) ? pre : lock_max(c);
+                        if (in) p = (pre != kNoPre && pre != INUSE) ? pre : lock_max_eager(c);
                         if (is_inshold(p)) p = DELMOD;
                         bc[BC_C + threadIdx.x] = p;
                     }

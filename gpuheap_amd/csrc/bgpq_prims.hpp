@@ -426,21 +426,33 @@ __device__ __forceinline__ void merge_path_at(const K* a, const K* b, int d, K (
         else hi = mid;
     }
     const int ai = lo, bi = d - lo;
-    // unconditional loads at clamped indexes, then a select: a guarded load
-    // would become a branch with its own wait per key
-    K x[L], y[L];
+    if (ai + L <= B && bi + L <= B) {
+        // both runs inside the inputs: plain loads at fixed offsets
+        K x[L], y[L];
 #pragma unroll
-    for (int s = 0; s < L; ++s) {
-        x[s] = a[kmin(ai + s, B - 1)];
-        y[s] = b[kmin(bi + s, B - 1)];
+        for (int s = 0; s < L; ++s) {
+            x[s] = a[ai + s];
+            y[s] = b[bi + s];
+        }
+#pragma unroll
+        for (int s = 0; s < L; ++s) out[s] = kmin(x[s], y[L - 1 - s]);
+    } else {
+        // unconditional loads at clamped indexes, then a select: a guarded
+        // load would become a branch with its own wait per key
+        K x[L], y[L];
+#pragma unroll
+        for (int s = 0; s < L; ++s) {
+            x[s] = a[kmin(ai + s, B - 1)];
+            y[s] = b[kmin(bi + s, B - 1)];
+        }
+#pragma unroll
+        for (int s = 0; s < L; ++s) {
+            x[s] = (ai + s < B) ? x[s] : key_max<K>();
+            y[s] = (bi + s < B) ? y[s] : key_max<K>();
+        }
+#pragma unroll
+        for (int s = 0; s < L; ++s) out[s] = kmin(x[s], y[L - 1 - s]);
     }
-#pragma unroll
-    for (int s = 0; s < L; ++s) {
-        x[s] = (ai + s < B) ? x[s] : key_max<K>();
-        y[s] = (bi + s < B) ? y[s] : key_max<K>();
-    }
-#pragma unroll
-    for (int s = 0; s < L; ++s) out[s] = kmin(x[s], y[L - 1 - s]);
 #pragma unroll
     for (int j = L / 2; j > 0; j >>= 1) {
 #pragma unroll
@@ -688,21 +700,36 @@ struct GroupMerge {
             hi = nhi;
         }
         const int ai = lo, bi = D - lo;
-        // unconditional loads at clamped indexes, then a select (a guarded
-        // load would become a branch with its own wait per key)
-        K x[LL], y[LL];
+        if (ai + W <= B && bi + W <= B) {
+            // the window lies inside both inputs (every group but a few near
+            // the ends): loads at fixed offsets from two base addresses
+            const K* xa = a + ai + j;
+            const K* yb = b + bi + (W - 1) - j - (LL - 1) * G;
+            K x[LL], y[LL];
 #pragma unroll
-        for (int s = 0; s < LL; ++s) {
-            const int p = s * G + j;
-            x[s] = a[kmin(ai + p, B - 1)];
-            y[s] = b[kmin(bi + W - 1 - p, B - 1)];
-        }
+            for (int s = 0; s < LL; ++s) {
+                x[s] = xa[s * G];
+                y[s] = yb[(LL - 1 - s) * G];
+            }
 #pragma unroll
-        for (int s = 0; s < LL; ++s) {
-            const int p = s * G + j;
-            const K xs = ai + p < B ? x[s] : key_max<K>();
-            const K ys = bi + W - 1 - p < B ? y[s] : key_max<K>();
-            out[s] = kmin(xs, ys);
+            for (int s = 0; s < LL; ++s) out[s] = kmin(x[s], y[s]);
+        } else {
+            // unconditional loads at clamped indexes, then a select (a guarded
+            // load would become a branch with its own wait per key)
+            K x[LL], y[LL];
+#pragma unroll
+            for (int s = 0; s < LL; ++s) {
+                const int p = s * G + j;
+                x[s] = a[kmin(ai + p, B - 1)];
+                y[s] = b[kmin(bi + W - 1 - p, B - 1)];
+            }
+#pragma unroll
+            for (int s = 0; s < LL; ++s) {
+                const int p = s * G + j;
+                const K xs = ai + p < B ? x[s] : key_max<K>();
+                const K ys = bi + W - 1 - p < B ? y[s] : key_max<K>();
+                out[s] = kmin(xs, ys);
+            }
         }
         // strides W/2 .. G: partner position p ^ stride is column j, row s ^ (stride/G)
 #pragma unroll
