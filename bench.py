@@ -366,7 +366,7 @@ def main():
     ap.add_argument("--variant", choices=sorted(G.VARIANTS), default="buitdd_np")
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--grid-insert", type=int, default=256)
-    ap.add_argument("--grid-delete", type=int, default=32)
+    ap.add_argument("--grid-delete", type=int, default=64)
     ap.add_argument("--order", choices=["random", "ascending", "descending"], default="random")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true")
