@@ -334,7 +334,8 @@ struct DevHeap {
             if constexpr (PB) {
                 g2l_batch<K, B, NT>(X1, node(0));
                 __syncthreads();
-                wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
+                if constexpr (kSplitFold) fold_large(X0, X1);
+                else wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
                 publish();
             }
             if (threadIdx.x == 0) {
@@ -345,6 +346,7 @@ struct DevHeap {
             }
             __syncthreads();
             if (aborted()) return false;
+            if constexpr (PB && kSplitFold) fold_small(X0, X1);
             pre_last_ = last;
         }
         K* dst = items + off;
@@ -427,13 +429,22 @@ struct DevHeap {
         if constexpr (PB) {
             g2l_batch<K, B, NT>(X1, node(0));
             __syncthreads();
-            wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
+            if constexpr (kSplitFold) fold_large(X0, X1);
+            else wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
             wait_stores();
             __syncthreads();
         }
         if (threadIdx.x == 0) release_st(0, AVAIL);
+        if constexpr (PB && kSplitFold) fold_small(X0, X1);
         sift_down(X0, X1, X2);
     }
+
+    // Smallest batch size whose level steps take the two merge halves one
+    // after the other; below it one full merge is cheaper (B = 1024: a
+    // merge-path half alone takes longer than the whole bitonic merge).
+    // Measured heap sort of 2^28 u32 keys at B = 2048: 2.33e8 keys/s split,
+    // 2.22e8 with one full group merge per step.
+    static constexpr int kSplitFrom = 2048;
 
     // bu: lanes 0 and 1 request the locks of the two children of `nxt`
     // without waiting; the state word found (INUSE if another workgroup holds
@@ -447,12 +458,30 @@ struct DevHeap {
         return kNoPre;
     }
 
+    // Folding the partial buffer (X1) into a batch (X0) for B >= kSplitFrom: the
+    // partial buffer's new contents, the larger half, are computed and stored
+    // first (fold_large), so status[0] can be released before the batch takes
+    // the smaller half (fold_small, from the same LDS inputs).
+    static constexpr bool kSplitFold = B >= kSplitFrom;
+    __device__ __forceinline__ void fold_large(const K* X0, const K* X1) const {
+        HalfMerge<K, B, NT, true> hi;
+        hi.run(X0, X1);
+        hi.put(ToGlobal<K>{node(0)});
+    }
+    __device__ __forceinline__ void fold_small(K* X0, const K* X1) const {
+        HalfMerge<K, B, NT, false> lo;
+        lo.run(X0, X1);
+        __syncthreads();  // every read of X0 is done
+        lo.put(ToLds<K>{X0});
+        __syncthreads();
+    }
+
     // ---- sift the candidate X0 down from the root, which the caller holds.
     // Each level merges the two children: the larger half goes back to the
     // child with the larger maximum (tgt), the smaller half competes with the
     // candidate (buitdd:422-443), and the sift continues at the other child
     // (oth).  The next deleter waits for the node this level releases, so for
-    // B >= 4096 the node's hold is kept short:
+    // B >= kSplitFrom the node's hold is kept short:
     //  - the smaller half is computed first; when the candidate is no smaller
     //    than all of it (the usual case) it is the node's new batch, and the
     //    node is published and released before the larger half is computed;
@@ -466,7 +495,7 @@ struct DevHeap {
     //    so the next level starts with its children in LDS.
     // Every wait is for a node deeper than every node held, as before.
     __device__ void sift_down(K* X0, K* X1, K* X2) const {
-        constexpr bool kSplit = B >= 4096;
+        constexpr bool kSplit = B >= kSplitFrom;
         constexpr bool kAhead = BU && kSplit;
         int* bc = bcast_slots();
         int cur = 1;
@@ -775,7 +804,10 @@ struct DevHeap {
                 return;
             }
             // 2.2: a full batch goes into the tree, the rest stays partial
-            wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
+            // (bu, B >= kSplitFrom: the batch takes its half after status[0] is
+            // released, below)
+            if constexpr (BU && kSplitFold) fold_large(X0, X1);
+            else wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
             if (threadIdx.x == 0) a_store(ctr(CTR_PARTIAL), pbs + size - B);
             publish();
         }
@@ -801,6 +833,7 @@ struct DevHeap {
                 flag_error(ERR_OVERFLOW);
                 return;
             }
+            if constexpr (PB && kSplitFold) fold_small(X0, X1);
             BGPQ_EVENT(23);
             l2g_batch<K, B, NT>(node(cur), X0);
             publish();
@@ -849,7 +882,7 @@ struct DevHeap {
                         publish();
                         if (threadIdx.x == 0) release_st(par, par_word);
                         l2g_batch<K, B, NT>(node(cur), X1);
-                    } else if constexpr (B < 4096) {
+                    } else if constexpr (B < kSplitFrom) {
                         // small batches: one full merge is cheaper than two halves
                         wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(par)}, ToGlobal<K>{node(cur)});
                         publish();

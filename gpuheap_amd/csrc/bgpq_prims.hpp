@@ -854,11 +854,13 @@ __device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& s
 // can be taken from the same inputs one after the other, and the first can
 // be stored (and its node released) before the second is computed.  Thread
 // 0's v[0] is the half's smallest key and thread NT-1's v[N-1] its largest.
-// B = 8192 uses the group merge's windows, smaller B the merge path.
-template <typename K, int B, int NT, bool HI>
+// B >= 2048 uses the group merge's windows (one workgroup, u32, lower half:
+// 1.14 / 1.30 / 1.85 us at B = 2048 / 4096 / 8192 against 1.55 / 1.82 /
+// 2.51 us for the merge path), smaller B the merge path.
+template <typename K, int B, int NT, bool HI, bool GRP = (B >= 2048)>
 struct HalfMerge {
     static constexpr int N = B / NT;
-    static constexpr bool kGroup = B >= 8192;
+    static constexpr bool kGroup = GRP;
     static constexpr int G = kMergeGroup;
     static_assert(!kGroup || NT % G == 0, "half merge shape");
     K v[N];
@@ -882,16 +884,30 @@ struct HalfMerge {
 };
 
 // The merge used by the heap, by measurement on MI355X (one workgroup,
-// LDS inputs, u32 keys): the register bitonic merge up to B = 1024
-// (B = 1024, 512 threads: 0.80 us against 0.93 for the merge path), the
-// merge path with a 4-ary split search for B = 2048 and 4096 (B = 2048:
-// 1.18 us against 1.47 bitonic and 1.48 group), and the group merge at
-// B = 8192 (3.4 us against 4.9 merge path and 5.6 bitonic).
+// LDS inputs, u32 keys, small half to LDS and large half to global memory):
+//   B <= 1024: the register bitonic merge (B = 1024, 512 threads: 0.80 us
+//     against 0.93 for the merge path);
+//   B = 2048, 4096: the group merge (1.35 / 1.91 us against 1.59 / 2.24 for
+//     the merge path);
+//   B = 8192: the two group-window halves one after the other (3.55 us
+//     against 4.52 for the one-pass group merge, whose 16 keys per lane
+//     crowd the registers).
 template <typename K, int B, int NT, typename DS, typename DL>
 __device__ __forceinline__ void wg_merge(K* a, K* b, const DS& small, const DL& large) {
-    if constexpr (B >= 8192) wg_merge_grp<K, B, NT>(a, b, small, large);
-    else if constexpr (B >= 2048) wg_merge_mp<K, B, NT>(a, b, small, large);
-    else wg_merge_bitonic<K, B, NT>(a, b, small, large);
+    if constexpr (B >= 8192) {
+        HalfMerge<K, B, NT, false> lo;
+        lo.run(a, b);
+        HalfMerge<K, B, NT, true> hi;
+        hi.run(a, b);
+        __syncthreads();  // destinations may alias a or b
+        lo.put(small);
+        hi.put(large);
+        __syncthreads();
+    } else if constexpr (B >= 2048) {
+        wg_merge_grp<K, B, NT>(a, b, small, large);
+    } else {
+        wg_merge_bitonic<K, B, NT>(a, b, small, large);
+    }
 }
 
 }  // namespace bgpq

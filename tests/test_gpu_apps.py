@@ -22,7 +22,7 @@ def _heap(B, keys_hint, variant):
 
 
 @pytest.mark.parametrize("variant", ["buitdd", "tditdd"])
-@pytest.mark.parametrize("B", [256, 1024])
+@pytest.mark.parametrize("B", [256, 1024, 2048])
 @pytest.mark.parametrize("scale,seed", [(10, 0), (14, 1)])
 def test_sssp_matches_dijkstra(gpu, variant, B, scale, seed):
     print(f"[case] sssp {variant} B={B} scale={scale} seed={seed}", flush=True)
@@ -50,7 +50,7 @@ def test_sssp_disconnected(gpu):
 
 
 @pytest.mark.parametrize("variant", ["buitdd", "tditdd"])
-@pytest.mark.parametrize("B", [256, 1024])
+@pytest.mark.parametrize("B", [256, 1024, 2048])
 @pytest.mark.parametrize("size,p,seed", [(96, 0.25, 0), (256, 0.25, 1), (200, 0.35, 2)])
 def test_astar_matches_cpu(gpu, variant, B, size, p, seed):
     print(f"[case] astar {variant} B={B} {size}x{size} p={p} seed={seed}", flush=True)
@@ -74,7 +74,7 @@ def test_astar_unreachable_goal(gpu):
 
 
 @pytest.mark.parametrize("variant", ["buitdd", "tditdd"])
-@pytest.mark.parametrize("B", [256, 1024])
+@pytest.mark.parametrize("B", [256, 1024, 2048])
 @pytest.mark.parametrize("n,seed", [(40, 0), (120, 1), (200, 2)])
 def test_knapsack_matches_dp(gpu, variant, B, n, seed):
     print(f"[case] knapsack {variant} B={B} n={n} seed={seed}", flush=True)
