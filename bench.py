@@ -364,9 +364,11 @@ def main():
     ap.add_argument("--n", type=parse_n, default=1 << 28, help="keys (or log2 of keys if < 64)")
     ap.add_argument("--key", choices=sorted(KEYS), default="u32")
     ap.add_argument("--variant", choices=sorted(G.VARIANTS), default="buitdd_np")
-    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--batch", type=int, default=None,
+                    help="batch size (heap sort: 8192; --app: sssp 2048, astar 256, knapsack 256)")
     ap.add_argument("--grid-insert", type=int, default=256)
-    ap.add_argument("--grid-delete", type=int, default=64)
+    ap.add_argument("--grid-delete", type=int, default=None,
+                    help="delete workgroups (heap sort: 64) or driver workgroups (--app: sssp 64, astar 16, knapsack 64)")
     ap.add_argument("--order", choices=["random", "ascending", "descending"], default="random")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true")
@@ -395,11 +397,18 @@ def main():
     if args.app == "mixed":
         sys.exit(mixed_bench(args))
     if args.app:
+        # per-driver defaults: the fastest measured configuration of each solve
+        # (R-MAT scale 22, 4096^2 grid, n = 200) on one MI355X
+        dflt = {"sssp": (2048, 64), "astar": (256, 16), "knapsack": (256, 64)}.get(args.app, (1024, 64))
+        args.batch = args.batch or dflt[0]
+        args.grid_delete = args.grid_delete or dflt[1]
         if args.variant not in ("buitdd", "tditdd"):
             args.variant = "buitdd"
         if args.batch > 2048:
             args.batch = 1024
         sys.exit(app_bench(args))
+    args.batch = args.batch or 8192
+    args.grid_delete = args.grid_delete or 64
 
     if args.sweep:
         # the heap-sort sweep: every variant and batch size on three key sets,
