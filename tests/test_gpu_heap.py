@@ -71,11 +71,16 @@ def test_single_workgroup_state_parity(gpu, variant, B, key):
     assert np.array_equal(got, m.out[:dc])
 
 
-def _harness(variant, B, n, key, seed, grid):
+def _harness(variant, B, n, key, seed, grid, distinct=0):
     """test/heap_test.cu main (:85-172) and test/partial_heap_test.cu: returns
-    (heap output, expected, check-after-insert)."""
+    (heap output, expected, check-after-insert).  distinct > 0 folds each
+    half of the u32 keys onto that many values (ties everywhere), keeping the
+    harness's split: every key of the second half, which is inserted during
+    the concurrent phase, stays above every key of the first."""
     if key == "u32":
         keys = O.harness_keys(seed, n)
+        if distinct:
+            keys = (keys & np.uint32(1 << 31)) | (keys % np.uint32(distinct))
     else:
         k32 = O.harness_keys(seed, n).astype(np.uint64)
         keys = (k32 << np.uint64(32)) | np.arange(n, dtype=np.uint64)
@@ -238,5 +243,19 @@ def test_concurrent_stress(gpu, variant, B):
     for seed in range(4):
         print(f"[case] stress {variant} B={B} seed={seed}", flush=True)
         got, want, (ok, msg) = _harness(variant, B, 1 << 16, "u32", seed=100 + seed, grid=128)
+        assert ok, msg
+        assert np.array_equal(got, want), f"seed {seed}"
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("B", [2048, 8192])
+def test_concurrent_stress_large(gpu, variant, B):
+    """The large-batch sift pipeline (half merges, locks requested ahead,
+    children prefetched, the larger child released one level later) under
+    concurrent insertions and deletions, on several seeds, once with keys
+    folded onto 7 values so that merges and stop tests meet ties."""
+    for seed, distinct in ((200, 0), (201, 0), (202, 7)):
+        print(f"[case] stress large {variant} B={B} seed={seed} distinct={distinct}", flush=True)
+        got, want, (ok, msg) = _harness(variant, B, 1 << 22, "u32", seed=seed, grid=64, distinct=distinct)
         assert ok, msg
         assert np.array_equal(got, want), f"seed {seed}"
