@@ -278,10 +278,15 @@ struct DevHeap {
                                           __HIP_MEMORY_SCOPE_AGENT);
                     mode = 0;
                 }
-                // Only a batch below the root's children may leave early: the
+                // Only a batch below the root's children may leave early (the
                 // previous deleter may still be refilling the root from slots
-                // 2 and 3, while any deeper batch only holds keys >= its
-                // level-1 ancestor, hence >= everything the new root receives.
+                // 2 and 3).  The batch taken usually holds keys >= everything
+                // the new root receives, but not always: a deleter further
+                // ahead may still be refilling its ancestor from its children,
+                // and once the batch has left, that refill draws on the
+                // sibling alone.  The heap without the batch stays consistent
+                // (the slot reads as empty), so the root is then the smallest
+                // batch of the rest, and the root step below merges the two.
                 int last = 0;
                 if (mode == 2 && prefetch && nb >= 4) {
                     last = nb;
@@ -349,16 +354,40 @@ struct DevHeap {
             if constexpr (PB && kSplitFold) fold_small(X0, X1);
             pre_last_ = last;
         }
+        // the root leaves; its largest key is kept for the check below
         K* dst = items + off;
         if (((uintptr_t)dst % 16) == 0) {
-            g2g_batch<K, B, NT>(dst, node(1));
+            g2g_batch_last<K, B, NT>(dst, node(1), key_slot<K>());
         } else {
-            for (int i = threadIdx.x; i < B; i += NT) dst[i] = gload_key(node(1) + i);
+            for (int i = threadIdx.x; i < B; i += NT) {
+                const K v = gload_key(node(1) + i);
+                dst[i] = v;
+                if (i == B - 1) *key_slot<K>() = v;
+            }
+        }
+        __syncthreads();
+        if (last > 0 && X0_min(smemOffset) < *key_slot<K>()) {
+            // the rare case: the B smallest of root and candidate leave
+            K* X0 = lds_ws(smemOffset);
+            K* X1 = X0 + B;
+            wait_stores();  // the root's copy lands before the merge overwrites it
+            g2l_batch<K, B, NT>(X1, node(1));
+            merge_root_rare(X0, X1, X1 + B, dst);
         }
         size = B;
-        __syncthreads();
         BGPQ_EVENT(6);
         return true;
+    }
+
+    // The rare root step: dst <- the B smallest of the candidate X0 and the
+    // root X1, X0 <- the B largest.  A merge with few registers on purpose:
+    // it shares the kernel with the sift's merges.
+    __device__ __forceinline__ K X0_min(int smemOffset) const { return lds_ws(smemOffset)[0]; }
+    __device__ void merge_root_rare(K* X0, const K* X1, K* X2, K* dst) const {
+        for (int i = threadIdx.x; i < B; i += NT) X2[i] = X0[i];
+        __syncthreads();
+        wg_merge_lean<K, B, NT>(X2, X1, dst, X0);
+        __syncthreads();
     }
 
     // ----------------------------------------------------------- deleteUpdate
@@ -448,7 +477,9 @@ struct DevHeap {
 
     // bu: lanes 0 and 1 request the locks of the two children of `nxt`
     // without waiting; the state word found (INUSE if another workgroup holds
-    // it), or kNoPre.  The next level's lock phase takes the result.
+    // it), or kNoPre.  The next level's lock phase takes the result.  (td
+    // variants measured no faster with it: their insert targets make a
+    // request fail more often.)
     static constexpr int kNoPre = INT_MIN;
     __device__ __forceinline__ int lock_ahead(int nxt) const {
         if (threadIdx.x < 2) {
@@ -456,6 +487,9 @@ struct DevHeap {
             if (c <= h.batch_num) return a_max(st(c), INUSE);
         }
         return kNoPre;
+    }
+    __device__ __forceinline__ static bool held_ahead(int pre) {
+        return pre != kNoPre && pre != INUSE;
     }
 
     // Folding the partial buffer (X1) into a batch (X0) for B >= kSplitFrom: the
@@ -521,7 +555,7 @@ struct DevHeap {
                         const int c = threadIdx.x == 0 ? l : r;
                         const bool in = threadIdx.x == 0 ? lin : rin;
                         int p = AVAIL;
-                        if (in) p = (pre != kNoPre && pre != INUSE) ? pre : lock_max_eager(c);
+                        if (in) p = held_ahead(pre) ? pre : lock_max_eager(c);
                         if (is_inshold(p)) p = DELMOD;
                         bc[BC_C + threadIdx.x] = p;
                     }
@@ -620,8 +654,7 @@ struct DevHeap {
                         const int nl = left_child(oth), nr = right_child(oth);
                         plp = bc[BC_C];
                         prp = bc[BC_D];
-                        fetch = nl <= h.batch_num && nr <= h.batch_num && plp != kNoPre && plp != INUSE &&
-                                prp != kNoPre && prp != INUSE;
+                        fetch = nl <= h.batch_num && nr <= h.batch_num && held_ahead(plp) && held_ahead(prp);
                         if (fetch) {
                             fl.issue(node(nl));
                             fr.issue(node(nr));
@@ -678,7 +711,7 @@ struct DevHeap {
                 }
                 if constexpr (kAhead) {
                     // give back the locks requested ahead, untouched
-                    if (threadIdx.x < 2 && pre != kNoPre && pre != INUSE)
+                    if (threadIdx.x < 2 && held_ahead(pre))
                         a_store(st(threadIdx.x == 0 ? left_child(oth) : right_child(oth)), pre);
                 }
                 if (!stop) flag_error(ERR_LOST_KEYS);

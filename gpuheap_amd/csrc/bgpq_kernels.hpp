@@ -94,7 +94,8 @@ __global__ __launch_bounds__(threads_for(B)) void k_prim_sort(const K* in, const
 }
 
 // algo 0: the register bitonic merge, 1: the LDS merge path, 2: the group
-// merge, 3: the serial merge path, 4: the two halves one after the other
+// merge, 3: the serial merge path, 4: the two halves one after the other,
+// 5: the lean merge path of the delete's rare root step
 template <typename K, int B, int ALGO>
 __global__ __launch_bounds__(threads_for(B)) void k_prim_merge(const K* a, const K* b, K* small, K* large) {
     constexpr int NT = threads_for(B);
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(threads_for(B)) void k_prim_merge(const K* a, const
     else if constexpr (ALGO == 1) wg_merge_mp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
     else if constexpr (ALGO == 2) wg_merge_grp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
     else if constexpr (ALGO == 3) wg_merge_ser<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
+    else if constexpr (ALGO == 5) wg_merge_lean<K, B, NT>(X0, X1, small + o, large + o);
     else {
         // the two halves one after the other, as the heap's sift steps take them
         HalfMerge<K, B, NT, false> lo;
@@ -216,7 +218,8 @@ struct Launch {
             case 1: return merge_as<1>(a, b, small, large, batches, s);
             case 2: return merge_as<2>(a, b, small, large, batches, s);
             case 3: return merge_as<3>(a, b, small, large, batches, s);
-            default: return merge_as<4>(a, b, small, large, batches, s);
+            case 4: return merge_as<4>(a, b, small, large, batches, s);
+            default: return merge_as<5>(a, b, small, large, batches, s);
         }
     }
     static LaunchRow row() { return LaunchRow{B, &sort, &merge, &insert, &del, &conc}; }

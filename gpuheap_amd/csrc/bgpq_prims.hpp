@@ -214,6 +214,26 @@ __device__ __forceinline__ void g2g_batch(K* __restrict__ dst, const K* __restri
     }
 }
 
+// g2g_batch that also leaves the batch's last key in *last (LDS)
+template <typename K, int B, int NT>
+__device__ __forceinline__ void g2g_batch_last(K* __restrict__ dst, const K* __restrict__ g, K* last) {
+    constexpr int NV = B * (int)sizeof(K) / 16;
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(g);
+    uint4* d = reinterpret_cast<uint4*>(dst);
+#pragma unroll
+    for (int k = 0; k < (NV + NT - 1) / NT; ++k) {
+        const int c = threadIdx.x + k * NT;
+        if (NV % NT == 0 || c < NV) {
+            const uint4 v = gload16(r, c * 16);
+            d[c] = v;
+            if (c == NV - 1) {
+                if constexpr (sizeof(K) == 4) *last = (K)v.w;
+                else *last = (K)(((unsigned long long)v.w << 32) | v.z);
+            }
+        }
+    }
+}
+
 template <typename K>
 __device__ __forceinline__ uint4 splat16(K v) {
     uint4 f;
@@ -474,6 +494,35 @@ __device__ __forceinline__ void wg_merge_path_fast(const K* a, const K* b, K (&o
 
 // Destination tags for the halves of a merge: LDS, global heap node
 // (coherent stores), or dropped.
+// Merge of two sorted LDS batches with few registers: each thread finds the
+// co-rank of its own run of 2B/NT outputs by a binary search and merges that
+// run serially (a first among equal keys).  Output i < B goes to small[i],
+// the others to large[i - B], by plain stores; neither may alias a or b.
+// No barrier inside: the caller orders the inputs' stores before the call
+// and the outputs' use after it.
+template <typename K, int B, int NT>
+__device__ void wg_merge_lean(const K* a, const K* b, K* small, K* large) {
+    constexpr int PT = 2 * B / NT;
+    const int d0 = (int)threadIdx.x * PT;
+    int lo = d0 > B ? d0 - B : 0, hi = d0 < B ? d0 : B;
+    while (lo < hi) {  // first i with b[d0 - i - 1] < a[i]
+        const int mid = (lo + hi) >> 1;
+        if (!(b[d0 - mid - 1] < a[mid])) lo = mid + 1;
+        else hi = mid;
+    }
+    int i = lo, j = d0 - lo;
+    for (int k = 0; k < PT; ++k) {
+        const K x = a[i < B ? i : B - 1], y = b[j < B ? j : B - 1];
+        const bool ta = i < B && (j >= B || !(y < x));
+        const K v = ta ? x : y;
+        i += ta;
+        j += !ta;
+        const int d = d0 + k;
+        if (d < B) small[d] = v;
+        else large[d - B] = v;
+    }
+}
+
 struct ToNone {};
 template <typename K> struct ToLds { K* p; };
 template <typename K> struct ToGlobal { K* p; };

@@ -108,7 +108,8 @@ int bgpq_default_grid(int batch_size);
  *   the register bitonic merge, 1 the merge path with a 4-ary split search,
  *   2 the group merge, 3 the serial branch-free merge path (the heap picks
  *   one of them by batch size), 4 the two halves computed one after the
- *   other from the same inputs, as the heap's sift steps do. */
+ *   other from the same inputs, as the heap's sift steps do, 5 the
+ *   low-register merge path of the delete's rare root step. */
 int bgpq_test_sort(int key_type, int batch_size, const void* d_in, const int* d_sizes, void* d_out,
                    uint64_t pad, int batches);
 int bgpq_test_merge(int key_type, int batch_size, const void* d_a, const void* d_b, void* d_small, void* d_large,

@@ -48,7 +48,7 @@ def test_sort_batch(gpu, B, key):
 
 @pytest.mark.parametrize("B", BATCHES)
 @pytest.mark.parametrize("key", ["u32", "u64"])
-@pytest.mark.parametrize("algo", [0, 1, 2, 3, 4], ids=["bitonic", "merge_path", "group", "serial", "halves"])
+@pytest.mark.parametrize("algo", [0, 1, 2, 3, 4, 5], ids=["bitonic", "merge_path", "group", "serial", "halves", "lean"])
 def test_merge_batches(gpu, B, key, algo):
     kt, dtype = KEYS[key]
     if key == "u64" and B == 8192:
