@@ -492,6 +492,74 @@ struct DevHeap {
         return pre != kNoPre && pre != INUSE;
     }
 
+    // bu, B >= kSplitFrom: the lock phase of a level step whose children are
+    // not in LDS yet.  Lanes 0 and 1 request the two locks (or take the ones
+    // requested ahead) until at least one is held; the workgroup then starts
+    // loading that child while lane 0 or 1 waits for the other, which is
+    // loaded after a second barrier.  The deleter ahead releases the larger
+    // child about one publish before the node it descends from, so usually
+    // only the last child's load is left after the wait.  lp, rp: the state
+    // words found.  Returns false on abort.
+    __device__ bool lock_load_split(int l, int r, bool lin, bool rin, int pre, K* X1, K* X2, int& lp,
+                                    int& rp) const {
+        int* bc = bcast_slots();
+        bool need = false;  // lanes 0, 1: the lock is still to be taken
+        const int c = threadIdx.x == 0 ? l : r;
+        if (threadIdx.x < 64) {
+            int word = AVAIL;
+            if (threadIdx.x < 2 && (threadIdx.x == 0 ? lin : rin)) {
+                if (held_ahead(pre)) word = pre;
+                else need = true;
+            }
+            int n = 0;
+            while (true) {
+                if (need) {
+                    const int q = a_max(st(c), INUSE);
+                    if (q != INUSE) {
+                        word = q;
+                        need = false;
+                    } else if (poll_abort(n)) {
+                        word = -1;
+                        need = false;
+                    }
+                }
+                if (__ballot(threadIdx.x < 2 && !need) != 0) break;
+            }
+            if (threadIdx.x < 2) bc[BC_C + threadIdx.x] = need ? kNoPre : (is_inshold(word) ? DELMOD : word);
+        }
+        __syncthreads();
+        if (aborted()) return false;
+        const int w0 = bc[BC_C], w1 = bc[BC_D];
+        const bool h0 = w0 != kNoPre, h1 = w1 != kNoPre;
+        BatchLoad<K, B, NT> f0, f1;
+        if (h0 && lin) f0.issue(node(l));
+        if (h1 && rin) f1.issue(node(r));
+        lp = w0;
+        rp = w1;
+        if (!(h0 && h1)) {
+            if (threadIdx.x < 2 && need) {
+                const int q = lock_max_eager(c);
+                bc[BC_E + threadIdx.x] = is_inshold(q) ? DELMOD : q;
+            }
+            __syncthreads();
+            if (aborted()) return false;
+            if (!h0) {
+                lp = bc[BC_E];
+                f0.issue(node(l));
+            }
+            if (!h1) {
+                rp = bc[BC_F];
+                f1.issue(node(r));
+            }
+        }
+        if (lin) f0.commit(X1);
+        else fill_lds<K, B, NT>(X1, h.init_limits);
+        if (rin) f1.commit(X2);
+        else fill_lds<K, B, NT>(X2, h.init_limits);
+        __syncthreads();
+        return true;
+    }
+
     // Folding the partial buffer (X1) into a batch (X0) for B >= kSplitFrom: the
     // partial buffer's new contents, the larger half, are computed and stored
     // first (fold_large), so status[0] can be released before the batch takes
@@ -547,6 +615,10 @@ struct DevHeap {
                 ready = false;
                 lp = ready_lp;
                 rp = ready_rp;
+                type = 2;
+            } else if constexpr (BU && kSplit) {
+                if (!lock_load_split(l, r, lin, rin, pre, X1, X2, lp, rp)) return;
+                pre = kNoPre;
                 type = 2;
             } else {
                 if constexpr (BU) {
@@ -636,7 +708,15 @@ struct DevHeap {
                 if constexpr (kAhead) {
                     if (threadIdx.x < 2) bc[BC_C + threadIdx.x] = pre;
                 }
+                // the larger child stored one level up is released here, once
+                // acknowledged: the next deleter can load it while it waits
+                // for this node
+                if (pend) wait_stores();
                 __syncthreads();
+                if (pend) {
+                    if (threadIdx.x == 0) a_store(st(pend), pend_word);
+                    pend = 0;
+                }
                 BGPQ_EVENT(13);
                 if (!(cmin < *key_slot<K>()) && oth_in) {
                     // candidate >= all of the small half: the node takes it
