@@ -9,7 +9,8 @@
 // Every thread of the workgroup must call them.  `smemOffset` counts ints
 // into the kernel's dynamic LDS (`extern __shared__ int[]`), exactly as in
 // the reference; it must be a multiple of 4.  LDS needed past the offset:
-// insertion 2*B*sizeof(K), deleteUpdate 3*B*sizeof(K).
+// insertion 2*B*sizeof(K) (3*B*sizeof(K) for B >= 2048), deleteUpdate
+// 3*B*sizeof(K).
 //
 // Storage (HBM): items[(batch_num+1)*B]: slot 0 is the partial buffer,
 // slots 1..batchCount the full sorted batches, every other slot holds
@@ -98,7 +99,7 @@ __device__ __forceinline__ K* key_slot() {
 }
 
 template <typename K, int B>
-constexpr int insert_lds_bytes() { return 2 * B * (int)sizeof(K); }
+constexpr int insert_lds_bytes() { return (B >= 2048 ? 3 : 2) * B * (int)sizeof(K); }
 template <typename K, int B>
 constexpr int delete_lds_bytes() { return 3 * B * (int)sizeof(K); }
 
@@ -474,6 +475,10 @@ struct DevHeap {
     // Measured heap sort of 2^28 u32 keys at B = 2048: 2.33e8 keys/s split,
     // 2.22e8 with one full group merge per step.
     static constexpr int kSplitFrom = 2048;
+    // bu insertion walk for B >= kSplitFrom: the node's batch stays in LDS
+    // between steps (a third buffer) and the parent's load overlaps the
+    // node's lock
+    static constexpr bool kSplitWalk = BU && B >= kSplitFrom;
 
     // bu: lanes 0 and 1 request the locks of the two children of `nxt`
     // without waiting; the state word found (INUSE if another workgroup holds
@@ -959,23 +964,52 @@ struct DevHeap {
             // was pulled, and the walk may end as soon as parent.max <=
             // node.min: the merges above would be identities.
             bool dirty = false;
+            // B >= kSplitFrom: C holds the node's batch as this insertion last
+            // wrote it (reloaded only when a deleter touched the node), P the
+            // parent's, S the spare buffer that keeps the parent's new batch.
+            K* C = X0;
+            K* P = X1;
+            K* S = X1 + B;
             while (cur != 1) {
                 const int par = parent(cur);
-                if (threadIdx.x == 0) {
-                    int prev = -1;
-                    BGPQ_EVENT(24);
-                    if (lock_cas(par, AVAIL, INUSE)) prev = lock_max(cur);
-                    bc[BC_D] = prev;
+                int prev;
+                if constexpr (kSplitWalk) {
+                    // the parent is locked first (top-down order, as deleters
+                    // lock); its batch loads while lane 0 takes the node's lock
+                    if (threadIdx.x == 0) {
+                        BGPQ_EVENT(24);
+                        lock_cas(par, AVAIL, INUSE);
+                    }
+                    __syncthreads();
+                    if (aborted()) return;
+                    BatchLoad<K, B, NT> pl;
+                    pl.issue(node(par));
+                    if (threadIdx.x == 0) bc[BC_D] = lock_max(cur);
+                    __syncthreads();
+                    BGPQ_EVENT(25);
+                    if (aborted()) return;
+                    prev = bc[BC_D];
+                    pl.commit(P);
+                    if (prev != mine) g2l_batch<K, B, NT>(C, node(cur));
+                    __syncthreads();
+                } else {
+                    if (threadIdx.x == 0) {
+                        int p = -1;
+                        BGPQ_EVENT(24);
+                        if (lock_cas(par, AVAIL, INUSE)) p = lock_max(cur);
+                        bc[BC_D] = p;
+                    }
+                    __syncthreads();
+                    BGPQ_EVENT(25);
+                    if (aborted()) return;
+                    prev = bc[BC_D];
+                    g2l_batch<K, B, NT>(C, node(cur));
+                    g2l_batch<K, B, NT>(P, node(par));
+                    __syncthreads();
                 }
-                __syncthreads();
-                BGPQ_EVENT(25);
-                if (aborted()) return;
-                dirty = dirty || (bc[BC_D] != mine);
-                g2l_batch<K, B, NT>(X0, node(cur));
-                g2l_batch<K, B, NT>(X1, node(par));
-                __syncthreads();
                 BGPQ_EVENT(26);
-                const bool identity = !(X0[0] < X1[B - 1]);  // parent.max <= node.min
+                dirty = dirty || (prev != mine);
+                const bool identity = !(C[0] < P[B - 1]);  // parent.max <= node.min
                 const bool done = identity && !dirty;
                 const int par_word = (done || par == 1) ? AVAIL : mine;
                 if (identity) {
@@ -984,31 +1018,39 @@ struct DevHeap {
                         release_st(par, par_word);
                         release_st(cur, AVAIL);
                     }
+                    K* t = C;  // the parent's batch is unchanged
+                    C = P;
+                    P = t;
                 } else {
                     // The parent's new batch is stored and the parent released
                     // first (every insertion that reaches the root waits for
                     // it); the node stays locked until it holds the larger
                     // half, so no one reads it in between.
-                    if (!(X1[0] < X0[B - 1])) {
-                        // parent.min >= node.max: swap (buitdd:647-661)
-                        l2g_batch<K, B, NT>(node(par), X0);
+                    if (!(P[0] < C[B - 1])) {
+                        // parent.min >= node.max: swap (buitdd:647-661); the
+                        // parent now holds C
+                        l2g_batch<K, B, NT>(node(par), C);
                         publish();
                         if (threadIdx.x == 0) release_st(par, par_word);
-                        l2g_batch<K, B, NT>(node(cur), X1);
+                        l2g_batch<K, B, NT>(node(cur), P);
                     } else if constexpr (B < kSplitFrom) {
                         // small batches: one full merge is cheaper than two halves
-                        wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(par)}, ToGlobal<K>{node(cur)});
+                        wg_merge<K, B, NT>(C, P, ToGlobal<K>{node(par)}, ToGlobal<K>{node(cur)});
                         publish();
                         if (threadIdx.x == 0) release_st(par, par_word);
                     } else {
                         HalfMerge<K, B, NT, false> lo;
-                        lo.run(X0, X1);
+                        lo.run(C, P);
                         lo.put(ToGlobal<K>{node(par)});
+                        lo.put(ToLds<K>{S});
                         publish();
                         if (threadIdx.x == 0) release_st(par, par_word);
                         HalfMerge<K, B, NT, true> hi;
-                        hi.run(X0, X1);
+                        hi.run(C, P);
                         hi.put(ToGlobal<K>{node(cur)});
+                        K* t = C;  // the parent's new batch is S
+                        C = S;
+                        S = t;
                     }
                     publish();
                     if (threadIdx.x == 0) release_st(cur, AVAIL);
