@@ -724,10 +724,17 @@ struct DevHeap {
                 if constexpr (kAhead) {
                     if (threadIdx.x < 2) bc[BC_C + threadIdx.x] = pre;
                 }
+                // the node's new batch in the usual case (below), stored
+                // before the barrier; the rare paths overwrite it
+                if (oth_in) lo.put(ToGlobal<K>{node(cur)});
                 // the larger child stored one level up is released here, once
-                // acknowledged: the next deleter can load it while it waits
-                // for this node
-                if (pend) wait_stores();
+                // acknowledged (its stores are older than the node's, and
+                // only stores are in flight): the next deleter can load it
+                // while it waits for this node
+                if (pend) {
+                    if (oth_in) wait_vmem_but<HalfMerge<K, B, NT, false>::kGlobalStores>();
+                    else wait_stores();
+                }
                 __syncthreads();
                 if (pend) {
                     if (threadIdx.x == 0) a_store(st(pend), pend_word);
@@ -736,7 +743,7 @@ struct DevHeap {
                 BGPQ_EVENT(13);
                 if (!(cmin < *key_slot<K>()) && oth_in) {
                     // candidate >= all of the small half: the node takes it
-                    lo.put(ToGlobal<K>{node(cur)});
+                    // (already stored above)
                     publish();
                     if (threadIdx.x == 0) {
                         release_st(cur, BU ? cur_prev : AVAIL);
@@ -776,7 +783,10 @@ struct DevHeap {
                     cur_prev = oth_prev;
                     continue;
                 }
-                // the candidate takes part: both halves, the small one into X1
+                // the candidate takes part: both halves, the small one into X1.
+                // The node is written again below: the batch stored above
+                // lands first.
+                wait_stores();
                 HalfMerge<K, B, NT, true> hi;
                 hi.run(X1, X2);
                 __syncthreads();  // every read of X1 and X2 is done

@@ -41,6 +41,14 @@ namespace bgpq {
 constexpr int kSC1 = 16;  // CPol::SC1 on gfx940+: device-coherent
 
 __device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
+// vmcnt(N): every vector-memory access of this lane but the N newest is done.
+// Stores retire in issue order, so with only stores in flight this waits for
+// the older ones while the N newest may still be on their way.
+template <int N>
+__device__ __forceinline__ void wait_vmem_but() {
+    static_assert(N >= 0 && N < 16, "vmcnt field");
+    __builtin_amdgcn_s_waitcnt(0x0F70 | N);
+}
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
@@ -849,6 +857,12 @@ __device__ __forceinline__ void quad_transpose(K (&v)[L]) {
 // To global memory with L % 4 == 0, a quad transpose first gives lane
 // 4q + r the four consecutive keys off + (4m + r)*G + 4q .. +3 of each block
 // m, which go out as 16-byte stores (u32; two for u64) instead of 4-byte ones.
+// Global store instructions put_strided issues per lane (kept in step with
+// its branches below).
+template <typename K, int L, int G>
+constexpr int strided_global_stores() {
+    return (L % 4 == 0 && G % 4 == 0) ? (sizeof(K) == 4 ? L / 4 : L / 2) : L;
+}
 template <typename K, int L, int G, typename D>
 __device__ __forceinline__ void put_strided(const D& dst, int off, int j, const K (&v)[L]) {
     if constexpr (std::is_same<D, ToNone>::value) {
@@ -930,6 +944,8 @@ struct HalfMerge {
         else put_run<K, N>(dst, t * N, v);
     }
     __device__ __forceinline__ K last() const { return v[N - 1]; }
+    // vector-memory store instructions one lane issues in put(ToGlobal)
+    static constexpr int kGlobalStores = kGroup ? strided_global_stores<K, N, G>() : -1;
 };
 
 // The merge used by the heap, by measurement on MI355X (one workgroup,
