@@ -1006,13 +1006,15 @@ struct DevHeap {
                     BatchLoad<K, B, NT> pl;
                     pl.issue(node(par));
                     if (threadIdx.x == 0) bc[BC_D] = lock_max(cur);
+                    pl.commit(P);
                     __syncthreads();
                     BGPQ_EVENT(25);
                     if (aborted()) return;
                     prev = bc[BC_D];
-                    pl.commit(P);
-                    if (prev != mine) g2l_batch<K, B, NT>(C, node(cur));
-                    __syncthreads();
+                    if (prev != mine) {  // a deleter touched the node: reload it
+                        g2l_batch<K, B, NT>(C, node(cur));
+                        __syncthreads();
+                    }
                 } else {
                     if (threadIdx.x == 0) {
                         int p = -1;
