@@ -125,6 +125,10 @@ struct DevHeap {
     // Set by deleteRoot when it already took the last batch into LDS (and
     // folded the partial buffer in): the slot it took, 0 otherwise.
     mutable int pre_last_ = 0;
+    // Set by deleteRoot: the partial buffer's size it found under status[0]
+    // (still held by deleteUpdate).  With an empty partial buffer (all
+    // init_limits) the fold into the candidate is the identity and is skipped.
+    mutable int pbs_ = 0;
 
     __device__ __forceinline__ K* node(int i) const { return h.items + (size_t)i * B; }
     __device__ __forceinline__ int* st(int i) const { return h.status + (size_t)i * kWordStride; }
@@ -318,6 +322,7 @@ struct DevHeap {
         __syncthreads();
         if (aborted()) return false;
         const int mode = bc[BC_A], pbs = bc[BC_B], off = bc[BC_C], last = bc[BC_D];
+        pbs_ = pbs;
         __syncthreads();
         if (mode == 0) {
             if (threadIdx.x == 0) release_st(0, AVAIL);
@@ -349,11 +354,13 @@ struct DevHeap {
             BGPQ_EVENT(4);
             if (threadIdx.x == 0) release_st(last, AVAIL);
             if constexpr (PB) {
-                g2l_batch<K, B, NT>(X1, node(0));
-                __syncthreads();
-                if constexpr (kSplitFold) fold_large(X0, X1);
-                else wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
-                publish();
+                if (pbs > 0) {
+                    g2l_batch<K, B, NT>(X1, node(0));
+                    __syncthreads();
+                    if constexpr (kSplitFold) fold_large(X0, X1);
+                    else wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
+                    publish();
+                }
             }
             if (threadIdx.x == 0) {
                 lock_cas(1, AVAIL, INUSE);
@@ -363,7 +370,9 @@ struct DevHeap {
             }
             __syncthreads();
             if (aborted()) return false;
-            if constexpr (PB && kSplitFold) fold_small(X0, X1);
+            if constexpr (PB && kSplitFold) {
+                if (pbs > 0) fold_small(X0, X1);
+            }
             pre_last_ = last;
         }
         // the root leaves; its largest key is kept for the check below
@@ -467,16 +476,21 @@ struct DevHeap {
         __syncthreads();
 
         // ---- fold the partial buffer in: the larger half stays there
+        const bool fold = PB && pbs_ > 0;
         if constexpr (PB) {
-            g2l_batch<K, B, NT>(X1, node(0));
-            __syncthreads();
-            if constexpr (kSplitFold) fold_large(X0, X1);
-            else wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
-            wait_stores();
-            __syncthreads();
+            if (fold) {
+                g2l_batch<K, B, NT>(X1, node(0));
+                __syncthreads();
+                if constexpr (kSplitFold) fold_large(X0, X1);
+                else wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
+                wait_stores();
+                __syncthreads();
+            }
         }
         if (threadIdx.x == 0) release_st(0, AVAIL);
-        if constexpr (PB && kSplitFold) fold_small(X0, X1);
+        if constexpr (PB && kSplitFold) {
+            if (fold) fold_small(X0, X1);
+        }
         sift_down(X0, X1, X2);
     }
 
@@ -924,31 +938,37 @@ struct DevHeap {
                 }
                 return;
             }
-            g2l_batch<K, B, NT>(X1, node(0));
-            __syncthreads();
-            if (size + pbs < B) {  // 2.1: fold into the partial buffer via the root
-                wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToNone{});
-                if (threadIdx.x == 0) lock_cas(1, AVAIL, INUSE);
+        }
+        // 2.2 with an empty partial buffer: the fold below is the identity
+        const bool fold = PB && !(pbs == 0 && size == B);
+        if constexpr (PB) {
+            if (fold) {
+                g2l_batch<K, B, NT>(X1, node(0));
                 __syncthreads();
-                if (aborted()) return;
-                g2l_batch<K, B, NT>(X1, node(1));
-                __syncthreads();
-                wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(1)}, ToGlobal<K>{node(0)});
-                publish();
-                if (threadIdx.x == 0) {
-                    a_store(ctr(CTR_PARTIAL), pbs + size);
-                    release_st(1, AVAIL);
-                    release_st(0, AVAIL);
+                if (size + pbs < B) {  // 2.1: fold into the partial buffer via the root
+                    wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToNone{});
+                    if (threadIdx.x == 0) lock_cas(1, AVAIL, INUSE);
+                    __syncthreads();
+                    if (aborted()) return;
+                    g2l_batch<K, B, NT>(X1, node(1));
+                    __syncthreads();
+                    wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(1)}, ToGlobal<K>{node(0)});
+                    publish();
+                    if (threadIdx.x == 0) {
+                        a_store(ctr(CTR_PARTIAL), pbs + size);
+                        release_st(1, AVAIL);
+                        release_st(0, AVAIL);
+                    }
+                    return;
                 }
-                return;
+                // 2.2: a full batch goes into the tree, the rest stays partial
+                // (bu, B >= kSplitFrom: the batch takes its half after
+                // status[0] is released, below)
+                if constexpr (BU && kSplitFold) fold_large(X0, X1);
+                else wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
+                if (threadIdx.x == 0) a_store(ctr(CTR_PARTIAL), pbs + size - B);
+                publish();
             }
-            // 2.2: a full batch goes into the tree, the rest stays partial
-            // (bu, B >= kSplitFrom: the batch takes its half after status[0] is
-            // released, below)
-            if constexpr (BU && kSplitFold) fold_large(X0, X1);
-            else wg_merge<K, B, NT>(X0, X1, ToLds<K>{X0}, ToGlobal<K>{node(0)});
-            if (threadIdx.x == 0) a_store(ctr(CTR_PARTIAL), pbs + size - B);
-            publish();
         }
 
         if constexpr (BU) {
@@ -972,7 +992,9 @@ struct DevHeap {
                 flag_error(ERR_OVERFLOW);
                 return;
             }
-            if constexpr (PB && kSplitFold) fold_small(X0, X1);
+            if constexpr (PB && kSplitFold) {
+                if (fold) fold_small(X0, X1);
+            }
             BGPQ_EVENT(23);
             l2g_batch<K, B, NT>(node(cur), X0);
             publish();
