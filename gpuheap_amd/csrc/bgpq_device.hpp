@@ -495,15 +495,16 @@ struct DevHeap {
     }
 
     // Smallest batch size whose level steps take the two merge halves one
-    // after the other; below it one full merge is cheaper (B = 1024: a
-    // merge-path half alone takes longer than the whole bitonic merge).
-    // Measured heap sort of 2^28 u32 keys at B = 2048: 2.33e8 keys/s split,
-    // 2.22e8 with one full group merge per step.
-    static constexpr int kSplitFrom = 2048;
-    // bu insertion walk for B >= kSplitFrom: the node's batch stays in LDS
-    // between steps (a third buffer) and the parent's load overlaps the
-    // node's lock
-    static constexpr bool kSplitWalk = BU && B >= kSplitFrom;
+    // after the other, with the rest of the pipeline below.  Measured heap
+    // sort of 2^28 u32 keys (buitdd_np): B = 1024 1.36e8 keys/s split against
+    // 1.26e8 with one bitonic merge per step; B = 512 and 256 are 1% and 4%
+    // slower split, where a half merge costs about as much as a whole one.
+    static constexpr int kSplitFrom = 1024;
+    // bu insertion walk for B >= 2048: two half merges per step, the node's
+    // batch stays in LDS between steps (a third buffer) and the parent's load
+    // overlaps the node's lock (at B = 1024 one bitonic merge per step and
+    // the plain walk measured 4% faster)
+    static constexpr bool kSplitWalk = BU && B >= 2048;
 
     // bu: lanes 0 and 1 request the locks of the two children of `nxt`
     // without waiting; the state word found (INUSE if another workgroup holds
@@ -1007,9 +1008,11 @@ struct DevHeap {
             // was pulled, and the walk may end as soon as parent.max <=
             // node.min: the merges above would be identities.
             bool dirty = false;
-            // B >= kSplitFrom: C holds the node's batch as this insertion last
+            // kSplitWalk: C holds the node's batch as this insertion last
             // wrote it (reloaded only when a deleter touched the node), P the
-            // parent's, S the spare buffer that keeps the parent's new batch.
+            // parent's, S the spare buffer (LDS past 2*B keys) that keeps the
+            // parent's new batch.  Otherwise both are reloaded every step and
+            // only the first 2*B keys of LDS are used.
             K* C = X0;
             K* P = X1;
             K* S = X1 + B;
@@ -1078,8 +1081,8 @@ struct DevHeap {
                         publish();
                         if (threadIdx.x == 0) release_st(par, par_word);
                         l2g_batch<K, B, NT>(node(cur), P);
-                    } else if constexpr (B < kSplitFrom) {
-                        // small batches: one full merge is cheaper than two halves
+                    } else if constexpr (!kSplitWalk) {
+                        // B < 2048: one full merge is cheaper than two halves here
                         wg_merge<K, B, NT>(C, P, ToGlobal<K>{node(par)}, ToGlobal<K>{node(cur)});
                         publish();
                         if (threadIdx.x == 0) release_st(par, par_word);

@@ -917,10 +917,10 @@ __device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& s
 // can be taken from the same inputs one after the other, and the first can
 // be stored (and its node released) before the second is computed.  Thread
 // 0's v[0] is the half's smallest key and thread NT-1's v[N-1] its largest.
-// B >= 2048 uses the group merge's windows (one workgroup, u32, lower half:
+// B >= 1024 uses the group merge's windows (one workgroup, u32, lower half:
 // 1.14 / 1.30 / 1.85 us at B = 2048 / 4096 / 8192 against 1.55 / 1.82 /
 // 2.51 us for the merge path), smaller B the merge path.
-template <typename K, int B, int NT, bool HI, bool GRP = (B >= 2048)>
+template <typename K, int B, int NT, bool HI, bool GRP = (B >= 1024)>
 struct HalfMerge {
     static constexpr int N = B / NT;
     static constexpr bool kGroup = GRP;
