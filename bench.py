@@ -271,8 +271,7 @@ def mixed_bench(args):
     else:
         table = M.autotune(cat, budget, log=lambda m: print(m, file=sys.stderr, flush=True))
         if args.retune:
-            M.save_table(table, os.path.join(REPO, "gpurun_out", "autotune_mixed.json")
-                         if os.path.isdir(os.path.join(REPO, "gpurun_out")) else tpath)
+            M.save_table(table, args.table_out or tpath)
     print(f"[bench] size-class table: {json.dumps(table)}", file=sys.stderr, flush=True)
     plain, plain_lat, plain_s = M.run_plain(cat, stream, log=lambda m: print(m, file=sys.stderr, flush=True))
     print(f"[bench] plain path: {len(stream) / plain_s:.1f} q/s", file=sys.stderr, flush=True)
@@ -382,6 +381,8 @@ def main():
     ap.add_argument("--budget-gib", type=float, default=64.0, help="mixed: HBM budget of the serve arena")
     ap.add_argument("--rate", type=float, default=0.0, help="mixed: Poisson rate (0: half the closed-loop rate)")
     ap.add_argument("--retune", action="store_true", help="mixed: run the autotuner instead of the committed table")
+    ap.add_argument("--table-out", default=None,
+                    help="mixed --retune: where to write the new table (default: profiles/autotune_mixed.json)")
     ap.add_argument("--app", choices=["sssp", "astar", "knapsack", "mixed"],
                     help="time one driver solve instead of the heap sort (--batch, --variant, --grid-delete apply)")
     ap.add_argument("--scale", type=int, default=22, help="SSSP: log2 of the vertex count")
@@ -391,7 +392,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding design is described in DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU heap sort would route keys to per-GPU heaps by sampled splitters "
+              "with one all-to-all, and the mixed stream would split its queries)", file=sys.stderr)
         sys.exit(2)
     print(f"[bench] device: {G.device_name()}", file=sys.stderr, flush=True)
     if args.app == "mixed":
