@@ -6,7 +6,10 @@
 //   void deleteUpdate(int smemOffset)                   (buitdd:348-482, tditdd:345-534)
 //   void insertion(const K* items, int size, int smemOffset)   (buitdd:484-690, tditdd:536-753)
 //   int  getItemCount()                                 (buitdd:206-211)
-// Every thread of the workgroup must call them.  `smemOffset` counts ints
+// Every thread of the workgroup must call them.  deleteUpdate must follow a
+// deleteRoot that returned true, on the same DevHeap object (it carries the
+// state deleteRoot found under status[0]: the batch taken early, the partial
+// buffer's size) and with the same smemOffset.  `smemOffset` counts ints
 // into the kernel's dynamic LDS (`extern __shared__ int[]`), exactly as in
 // the reference; it must be a multiple of 4.  LDS needed past the offset:
 // insertion 2*B*sizeof(K) (3*B*sizeof(K) for B >= 2048), deleteUpdate
