@@ -210,19 +210,8 @@ __device__ __forceinline__ void l2l_batch(K* __restrict__ dst, const K* __restri
     }
 }
 // global -> global (heap node to caller buffer), coherent loads
-template <typename K, int B, int NT>
-__device__ __forceinline__ void g2g_batch(K* __restrict__ dst, const K* __restrict__ g) {
-    constexpr int NV = B * (int)sizeof(K) / 16;
-    const __amdgpu_buffer_rsrc_t r = buf_rsrc(g);
-    uint4* d = reinterpret_cast<uint4*>(dst);
-#pragma unroll
-    for (int k = 0; k < (NV + NT - 1) / NT; ++k) {
-        const int c = threadIdx.x + k * NT;
-        if (NV % NT == 0 || c < NV) d[c] = gload16(r, c * 16);
-    }
-}
-
-// g2g_batch that also leaves the batch's last key in *last (LDS)
+// Global-to-global copy of a batch (a coherent read of a heap node into the
+// caller's buffer) that also leaves the batch's last key in *last (LDS)
 template <typename K, int B, int NT>
 __device__ __forceinline__ void g2g_batch_last(K* __restrict__ dst, const K* __restrict__ g, K* last) {
     constexpr int NV = B * (int)sizeof(K) / 16;
