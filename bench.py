@@ -367,7 +367,7 @@ def main():
                     help="batch size (heap sort: 8192; --app: sssp 2048, astar 256, knapsack 256)")
     ap.add_argument("--grid-insert", type=int, default=256)
     ap.add_argument("--grid-delete", type=int, default=None,
-                    help="delete workgroups (heap sort: 64) or driver workgroups (--app: sssp 64, astar 16, knapsack 64)")
+                    help="delete workgroups (heap sort: 64) or driver workgroups (--app: sssp 32, astar 4, knapsack 8)")
     ap.add_argument("--order", choices=["random", "ascending", "descending"], default="random")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true")
@@ -401,7 +401,9 @@ def main():
     if args.app:
         # per-driver defaults: the fastest measured configuration of each solve
         # (R-MAT scale 22, 4096^2 grid, n = 200) on one MI355X
-        dflt = {"sssp": (2048, 64), "astar": (256, 16), "knapsack": (256, 64)}.get(args.app, (1024, 64))
+        # measured best (batch size, workgroups) per driver: fewer workgroups
+        # pop fewer keys out of order, hence less stale and repeated work
+        dflt = {"sssp": (2048, 32), "astar": (256, 4), "knapsack": (256, 8)}.get(args.app, (1024, 64))
         args.batch = args.batch or dflt[0]
         args.grid_delete = args.grid_delete or dflt[1]
         if args.variant not in ("buitdd", "tditdd"):
