@@ -108,7 +108,9 @@ class PlainRunner:
     """One query at a time through the single-solve drivers, each on a heap
     kept for its kind (emptied by every solve)."""
 
-    def __init__(self, cat, sssp_B=1024, astar_B=256, knap_B=256, grid_sssp=64, grid_astar=16, grid_knap=64):
+    # workgroups per driver as measured best for single solves (fewer
+    # workgroups pop fewer keys out of order)
+    def __init__(self, cat, sssp_B=1024, astar_B=256, knap_B=256, grid_sssp=32, grid_astar=4, grid_knap=8):
         emax = max(g["dev"].m for g in cat.graphs.values())
         cmax = max(g["dev"].width * g["dev"].height for g in cat.grids.values())
         self.cat = cat
