@@ -618,8 +618,8 @@ __device__ __forceinline__ void wg_merge_mp(const K* a, const K* b, const DS& sm
 // Thread t keeps c[tP..tP+P) and d[tP..tP+P) in registers, P = B/NT: strides
 // below P stay in the lane, strides below 64P use DPP / ds_swizzle /
 // ds_bpermute, and larger ones go through LDS, reusing a and b as scratch.
-// No step depends on the key values, so the latency is fixed and the LDS
-// traffic is bank-conflict free.  Ends with a barrier.
+// No step depends on the key values, so the latency is fixed.  Ends with a
+// barrier.
 template <typename K, int P, int NT>
 __device__ __forceinline__ void xlane_minmax(K (&c)[P], K (&d)[P], int m, bool lower) {
 #pragma unroll
