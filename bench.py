@@ -472,10 +472,14 @@ def main():
         "roofline": roofline,
     }
     if args.calibrate:
+        # a plain elementwise kernel of known traffic (reads and writes 1 GiB)
+        # to scale the FETCH_SIZE / WRITE_SIZE counters
+        import torch
         nb = 1 << 30
-        src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
-        check(lib().bgpq_dev_copy(dst.ptr, src.ptr, nb, None))
-        G.sync()
+        src = torch.ones(nb // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 3, out=dst)
+        torch.cuda.synchronize()
         line["calibration_copy_bytes"] = nb
     if not args.no_cpu:
         print("[bench] CPU baseline (std::sort, std::priority_queue) ...", file=sys.stderr, flush=True)

@@ -35,6 +35,10 @@ SIGNATURES = {
     "bgpq_set_cursor": (c_int, [c_void_p, c_i64]),
     "bgpq_concurrent": (c_int, [c_void_p, c_void_p, c_i64, c_int, c_void_p, c_i64, c_i64, c_int, c_void_p]),
     "bgpq_heapsort": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int, c_int, c_float_p, c_float_p, c_void_p]),
+    "bgpq_view": (c_int, [c_void_p, c_void_p]),
+    "bgpq_vote_reset": (c_int, [c_void_p, c_int, c_void_p]),
+    "bgpq_terminated": (c_int, [c_void_p]),
+    "bgpq_clear": (c_int, [c_void_p, c_void_p]),
     "bgpq_snapshot": (c_int, [c_void_p, c_void_p, c_int_p, c_int_p]),
     "bgpq_error_flags": (c_int, [c_void_p, c_int_p]),
     "bgpq_threads": (c_int, [c_int]),
@@ -43,7 +47,6 @@ SIGNATURES = {
     "bgpq_test_sort": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_u64, c_int]),
     "bgpq_test_merge": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int]),
     "bgpq_gen_keys": (c_int, [c_int, c_u64, c_i64, c_void_p, c_void_p]),
-    "bgpq_dev_copy": (c_int, [c_void_p, c_void_p, c_i64, c_void_p]),
     "bgpq_last_error": (ctypes.c_char_p, []),
     "bgpq_dev_malloc": (c_int, [ctypes.POINTER(c_void_p), c_size_t]),
     "bgpq_dev_free": (c_int, [c_void_p]),
@@ -69,6 +72,8 @@ SIGNATURES = {
     "bgpq_server_create": (c_int, [ctypes.POINTER(c_void_p), c_i64, c_int, c_int]),
     "bgpq_server_destroy": (None, [c_void_p]),
     "bgpq_server_start": (c_int, [c_void_p]),
+    "bgpq_server_set_idle_limit": (c_int, [c_void_p, c_i64]),
+    "bgpq_server_live": (c_int, [c_void_p]),
     "bgpq_server_submit": (c_int, [c_void_p, c_void_p]),
     "bgpq_server_poll": (c_int, [c_void_p, c_void_p, c_int]),
     "bgpq_server_inflight": (c_int, [c_void_p]),
@@ -76,6 +81,13 @@ SIGNATURES = {
     "bgpq_server_close": (c_int, [c_void_p]),
     "bgpq_batch_run": (c_int, [c_void_p, c_int, c_i64, c_int, c_int, c_void_p, c_float_p]),
 }
+
+
+class View(ctypes.Structure):
+    """bgpq_view_t (include/bgpq.h)."""
+    _fields_ = [("items", c_void_p), ("status", c_void_p), ("ctr", c_void_p), ("init_limits", c_u64),
+                ("batch_num", ctypes.c_int32), ("batch_size", ctypes.c_int32), ("variant", ctypes.c_int32),
+                ("key_type", ctypes.c_int32), ("word_stride", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class AppStats(ctypes.Structure):

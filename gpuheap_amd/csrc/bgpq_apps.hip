@@ -28,11 +28,18 @@ __global__ __launch_bounds__(threads_for(B)) void k_knapsack(HeapView<uint64_t> 
 }
 
 // Control words to their start state, plus one 32-bit label set (the
-// source's distance, the start cell's g).
-__global__ void k_app_setup(long long* ctl, long long pool, uint32_t* label, uint32_t value) {
+// source's distance, the start cell's g), and the heap's termination vote
+// with `busy` workgroups counted busy (the seeding one, for a threshold
+// search).
+__global__ void k_app_setup(long long* ctl, long long pool, uint32_t* label, uint32_t value, int* heap_ctr,
+                            int busy) {
     const int i = threadIdx.x;
     if (i < APP_WORDS) ctl[(size_t)i * kCtlStride] = i == APP_OUTSTANDING ? 1 : (i == APP_POOL ? pool : 0);
     if (i == 0 && label) *label = value;
+    if (i == 0) {
+        heap_ctr[CTR_BUSY * kWordStride] = busy;
+        heap_ctr[CTR_TERM * kWordStride] = 0;
+    }
 }
 
 struct AppLaunch {
@@ -128,8 +135,10 @@ int check_heap(bgpq_heap* h, const AppLaunch** row) {
 int resolve_grid(int grid) { return grid > 0 ? grid : 64; }
 
 // Timed launch, then the control words and heap counters are read back.
+// A threshold search (A*) ends with its pruned keys still in the heap:
+// there must be exactly `outstanding` of them, and the heap is cleared.
 template <class F>
-int run_solve(bgpq_heap* h, hipStream_t s, bgpq_app_stats* stats, F launch) {
+int run_solve(bgpq_heap* h, hipStream_t s, bgpq_app_stats* stats, F launch, bool threshold = false) {
     hipEvent_t e0, e1;
     BGPQ_HIPCHK(hipEventCreate(&e0));
     BGPQ_HIPCHK(hipEventCreate(&e1));
@@ -152,17 +161,27 @@ int run_solve(bgpq_heap* h, hipStream_t s, bgpq_app_stats* stats, F launch) {
     int c[CTR_WORDS];
     int rc = bgpq_handle_counters(h, c);
     if (rc) return rc;
+    const long long left = threshold ? (long long)c[CTR_BATCH_COUNT] * h->batch_size + c[CTR_PARTIAL] : 0;
     if (stats) {
         stats->popped = word(APP_POPPED);
         stats->expanded = word(APP_EXPANDED);
         stats->stale = word(APP_STALE);
-        stats->pruned = word(APP_PRUNED);
+        stats->pruned = word(APP_PRUNED) + left;
         stats->pushed = word(APP_PUSHED);
         stats->error_flags = (int32_t)word(APP_ERROR) | (c[CTR_ERROR] << 8);
         stats->ms = ms;
     }
     if (word(APP_ERROR) || c[CTR_ERROR])
         return bgpq_fail(BGPQ_EHEAP, "driver error flags 0x%llx, heap error flags 0x%x", word(APP_ERROR), c[CTR_ERROR]);
+    if (threshold) {
+        if (word(APP_OUTSTANDING) != left || !c[CTR_TERM])
+            return bgpq_fail(BGPQ_EHEAP, "threshold search ended with %lld keys outstanding, %lld in the heap",
+                             word(APP_OUTSTANDING), left);
+        rc = bgpq_clear(h, s);
+        if (rc) return rc;
+        BGPQ_HIPCHK(hipStreamSynchronize(s));
+        return BGPQ_OK;
+    }
     if (word(APP_OUTSTANDING) != 0 || c[CTR_BATCH_COUNT] || c[CTR_PARTIAL])
         return bgpq_fail(BGPQ_EHEAP, "driver ended with %lld keys outstanding", word(APP_OUTSTANDING));
     return BGPQ_OK;
@@ -181,7 +200,7 @@ extern "C" int bgpq_sssp(bgpq_heap* h, const int32_t* d_rowptr, const int32_t* d
     hipStream_t s = (hipStream_t)stream;
     BGPQ_HIPCHK(fill_u32(d_dist, n_vertices, 0xFFFFFFFFu, s));
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_app_setup, dim3(1), dim3(64), 0, s, h->app_ctl, 0ll, d_dist + source, 0u);
+    hipLaunchKernelGGL(k_app_setup, dim3(1), dim3(64), 0, s, h->app_ctl, 0ll, d_dist + source, 0u, h->ctr, 0);
     BGPQ_HIPCHK(hipGetLastError());
     const SsspApp app{d_rowptr, d_col, d_w, d_dist, h->app_ctl};
     const HeapArgs a = bgpq_handle_args(h, 0);
@@ -203,14 +222,14 @@ extern "C" int bgpq_astar(bgpq_heap* h, const uint8_t* d_free, int32_t width, in
     const long long cells = (long long)width * height;
     BGPQ_HIPCHK(fill_u32(d_g, cells, 0xFFFFFFFFu, s));
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_app_setup, dim3(1), dim3(64), 0, s, h->app_ctl, 0ll, d_g, 0u);
+    hipLaunchKernelGGL(k_app_setup, dim3(1), dim3(64), 0, s, h->app_ctl, 0ll, d_g, 0u, h->ctr, 1);
     BGPQ_HIPCHK(hipGetLastError());
     const AstarApp app{d_free, width, height, d_g, h->app_ctl};
     const uint32_t dx = (uint32_t)(width - 1), dy = (uint32_t)(height - 1);
     const uint32_t h0 = 10u * std::max(dx, dy) + 4u * std::min(dx, dy);
     const HeapArgs a = bgpq_handle_args(h, 0);
     const int g = resolve_grid(grid);
-    rc = run_solve(h, s, stats, [&] { return row->astar(a, app, h->app_ctl, (uint64_t)h0 << 32, g, s); });
+    rc = run_solve(h, s, stats, [&] { return row->astar(a, app, h->app_ctl, (uint64_t)h0 << 32, g, s); }, true);
     if (rc) return rc;
     BGPQ_HIPCHK(hipMemcpy(cost, d_g + (cells - 1), sizeof(uint32_t), hipMemcpyDeviceToHost));
     return BGPQ_OK;
@@ -271,7 +290,7 @@ extern "C" int bgpq_knapsack(bgpq_heap* h, int32_t n, const uint32_t* weight, co
         hipMemcpyAsync(mem + o_pool, &root, sizeof root, hipMemcpyHostToDevice, s) != hipSuccess)
         return release(bgpq_fail(BGPQ_EHIP, "knapsack: upload failed"));
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_app_setup, dim3(1), dim3(64), 0, s, h->app_ctl, 1ll, nullptr, 0u);
+    hipLaunchKernelGGL(k_app_setup, dim3(1), dim3(64), 0, s, h->app_ctl, 1ll, nullptr, 0u, h->ctr, 0);
     if (hipGetLastError() != hipSuccess) return release(bgpq_fail(BGPQ_EHIP, "knapsack: setup launch"));
     const KnapsackApp app{n,
                           capacity,

@@ -10,7 +10,9 @@
 //
 // Loop of one workgroup:
 //   1. deleteRoot/deleteUpdate pops up to B keys into LDS (a cheap look at
-//      the heap's counters first keeps idle workgroups off status[0]);
+//      the heap's counters first keeps idle workgroups off status[0]); an
+//      App with kThreshold (A*) pops only below App::threshold(), the A*
+//      forks' deleteRoot (artifacts/astar/BGPQ_B/heap.cuh:260-287);
 //   2. App::degree gives each key its number of children (0 when the key is
 //      stale or pruned); a workgroup prefix scan turns the degrees into
 //      offsets;
@@ -19,7 +21,10 @@
 //      is inserted B keys at a time (the rest at the end of the batch);
 //   4. outstanding += children before they are inserted, -= popped after.
 // Keys are never lost: outstanding reaches 0 only once the heap is empty
-// and no workgroup holds keys.
+// and no workgroup holds keys.  A threshold search ends instead with the
+// heap's termination vote (PBS_MODEL, tditdd:282-289): every workgroup has
+// found nothing below the threshold since its last pop.  The keys left in
+// the heap then number exactly `outstanding`, all pruned.
 #pragma once
 #include "bgpq_device.hpp"
 #include "bgpq_kernels.hpp"
@@ -128,6 +133,7 @@ __device__ void drive(HeapView<uint64_t> h, const App& app, long long* ctl, uint
     __shared__ int s_scan[NT / 64 + 1];
     __shared__ AppStats st;
     DevHeap<K, VAR, B, NT> hp{h};
+    constexpr bool kThr = App::kThreshold;
     const int t = threadIdx.x;
     if (t == 0) {
         st = AppStats{0, 0, 0, 0, 0};
@@ -140,19 +146,27 @@ __device__ void drive(HeapView<uint64_t> h, const App& app, long long* ctl, uint
         __syncthreads();
         hp.insertion(popped, 1, kWs);
         if (t == 0) st.pushed = 1;
+        if (kThr) hp.voteBusy();  // counted busy by the solve's setup
     }
     long long idle = 0;
     while (!hp.aborted()) {
-        if (t == 0) s_word = (a_load(h.ctr + CTR_BATCH_COUNT * kWordStride) | a_load(h.ctr + CTR_PARTIAL * kWordStride)) != 0;
+        // a busy workgroup must reach deleteRoot to cast its idle vote
+        if (t == 0)
+            s_word = (a_load(h.ctr + CTR_BATCH_COUNT * kWordStride) | a_load(h.ctr + CTR_PARTIAL * kWordStride)) != 0 ||
+                     (kThr && hp.busy_);
         __syncthreads();
         int size = 0;
         if (s_word) {
-            if (hp.deleteRoot(popped, size, kWs)) hp.deleteUpdate(kWs);
+            if constexpr (kThr) {
+                if (hp.deleteRoot(popped, size, app.threshold(), kWs)) hp.deleteUpdate(kWs);
+            } else {
+                if (hp.deleteRoot(popped, size, kWs)) hp.deleteUpdate(kWs);
+            }
             if (hp.aborted()) break;
         }
         if (size == 0) {
             if (t == 0) {
-                s_word = ctl_load(ctl, APP_OUTSTANDING) == 0;
+                s_word = kThr ? a_load(h.ctr + CTR_TERM * kWordStride) != 0 : ctl_load(ctl, APP_OUTSTANDING) == 0;
                 if (!s_word && ++idle > kMaxIdlePolls) {
                     ctl_or(ctl, APP_ERROR, APPERR_IDLE);
                     s_word = 1;
@@ -254,6 +268,7 @@ __device__ void drive(HeapView<uint64_t> h, const App& app, long long* ctl, uint
 // whose dist is above the vertex's current label is stale (sssp.cu:48-57);
 // relaxations use atomicMin on the labels (sssp.cu:103-120).
 struct SsspApp {
+    static constexpr bool kThreshold = false;
     const int* rowptr;
     const int* col;
     const uint32_t* w;
@@ -287,10 +302,14 @@ struct SsspApp {
 // -------------------------------------------------------------------- A*
 // Keys f << 32 | cell on an 8-connected grid; moves cost 10 / 14 into any
 // free cell (artifacts/astar/pq_model.cuh:114-184 checks only the target
-// cell), octile heuristic.  A popped key whose g is above the cell's label
-// is stale; one whose f is not below the goal's label is pruned (the
-// threshold deleteRoot, artifacts/astar/BGPQ_B/heap.cuh:273-287).
+// cell), octile heuristic.  Pops stop at the goal's label: deleteRoot with
+// the threshold g[goal] << 32 hands out only keys whose f is below it (the
+// reference passes dist[target], pq_model.cuh:287-296, to
+// artifacts/astar/BGPQ_B/heap.cuh:273-287).  A popped key whose g is above
+// the cell's label is stale; one whose f has reached the goal's label since
+// it was popped is pruned.
 struct AstarApp {
+    static constexpr bool kThreshold = true;
     const uint8_t* free_cells;
     int width, height;
     uint32_t* g;
@@ -302,6 +321,7 @@ struct AstarApp {
         return 10u * hi + 4u * lo;
     }
     __device__ uint32_t goal() const { return (uint32_t)width * (uint32_t)height - 1u; }
+    __device__ uint64_t threshold() const { return (uint64_t)u32_load(g + goal()) << 32; }
     __device__ int degree(uint64_t key, AppStats& st) const {
         const uint32_t f = (uint32_t)(key >> 32), cell = (uint32_t)key;
         const uint32_t gc = f - h(cell);
@@ -355,6 +375,7 @@ __device__ __forceinline__ uint64_t knap_key(uint64_t ub, uint32_t level, uint32
 }
 
 struct KnapsackApp {
+    static constexpr bool kThreshold = false;
     int n;
     uint64_t cap;
     const uint32_t* w;      // item weights in value/weight order

@@ -65,14 +65,8 @@ static const LaunchRow* rows_for(int key_type, int variant) {
 static size_t status_bytes(int batch_num) { return sizeof(int) * kWordStride * (size_t)(batch_num + 1); }
 static size_t ctr_bytes() { return sizeof(int) * kWordStride * CTR_WORDS; }
 
-static int env_pack() {
-    const char* v = getenv("BGPQ_XCD_PACK");
-    const int p = v ? atoi(v) : 1;
-    return p >= 1 && p <= 8 ? p : 1;
-}
-
 static HeapArgs args_of(const bgpq_heap* h, long long out_cap = 0) {
-    return HeapArgs{h->items, h->status, h->ctr, h->init_limits, h->batch_num, 1, out_cap, env_pack()};
+    return HeapArgs{h->items, h->status, h->ctr, h->init_limits, h->batch_num, 1, out_cap};
 }
 
 
@@ -112,7 +106,8 @@ static int status_words(bgpq_heap* h, int* out, int count) {
 // checkInsertHeap restated (heap/buitdd/heap.cuh:118-172, heap/tditdd/heap.cuh:123-176):
 // every status word AVAIL, each batch ascending, each child's first key >=
 // its parent's last key, and, when both exist, the partial buffer ascending
-// and not below the root's last key.
+// and not below the root's last key.  The first violation is reported in
+// the same words, and found in the same order, as the oracle's orc_check.
 template <typename K>
 static int check_impl(bgpq_heap* h) {
     int c[CTR_WORDS];
@@ -120,9 +115,9 @@ static int check_impl(bgpq_heap* h) {
     if (rc) return rc;
     const int nb = c[CTR_BATCH_COUNT], pbs = c[CTR_PARTIAL];
     const int B = h->batch_size;
-    if (c[CTR_ERROR]) return fail(0, "device error flags 0x%x", c[CTR_ERROR]);
-    if (nb < 0 || nb > h->batch_num) return fail(0, "batch count %d out of range", nb);
-    if (pbs < 0 || pbs >= B) return fail(0, "partial buffer size %d out of range", pbs);
+    if (c[CTR_ERROR]) return fail(0, "error flags 0x%x", c[CTR_ERROR]);
+    if (nb < 0 || nb > h->batch_num) return fail(0, "batch count %d", nb);
+    if (pbs < 0 || pbs >= B) return fail(0, "partial size %d", pbs);
     std::vector<K> items((size_t)(nb + 1) * B);
     std::vector<int> status(nb + 1);
     HIPCHK(hipMemcpy(items.data(), h->items, sizeof(K) * items.size(), hipMemcpyDeviceToHost));
@@ -132,7 +127,7 @@ static int check_impl(bgpq_heap* h) {
     if (status[0] != 0) return fail(0, "partial batch state %d, expected AVAIL", status[0]);
     if (nb != 0 && pbs != 0) {
         if (items[(size_t)2 * B - 1] > items[0])
-            return fail(0, "partial buffer smaller than the root's last key");
+            return fail(0, "partial buffer below the root's last key");
         for (int i = 1; i < pbs; ++i)
             if (items[i] < items[i - 1]) return fail(0, "partial buffer not ascending at %d", i);
     }
@@ -141,7 +136,7 @@ static int check_impl(bgpq_heap* h) {
         if (i > 1) {
             const int p = bu ? (int)rev_idx(rev_idx((uint32_t)i) >> 1) : i / 2;
             if (items[(size_t)i * B] < items[(size_t)p * B + B - 1])
-                return fail(0, "batch %d's first key is below batch %d's last key", i, p);
+                return fail(0, "batch %d below parent %d", i, p);
         }
         for (int j = 1; j < B; ++j)
             if (items[(size_t)i * B + j] < items[(size_t)i * B + j - 1])
@@ -227,6 +222,50 @@ void bgpq_destroy(bgpq_heap* h) {
 int bgpq_reset(bgpq_heap* h, void* stream) {
     if (!h) return fail(BGPQ_EINVAL, "null handle");
     return fill_items(h, (hipStream_t)stream);
+}
+
+int bgpq_view(bgpq_heap* h, bgpq_view_t* out) {
+    if (!h || !out) return fail(BGPQ_EINVAL, "null argument");
+    out->items = h->items;
+    out->status = h->status;
+    out->ctr = h->ctr;
+    out->init_limits = h->init_limits;
+    out->batch_num = h->batch_num;
+    out->batch_size = h->batch_size;
+    out->variant = h->variant;
+    out->key_type = h->key_type;
+    out->word_stride = kWordStride;
+    out->reserved = 0;
+    return BGPQ_OK;
+}
+
+int bgpq_vote_reset(bgpq_heap* h, int busy, void* stream) {
+    if (!h || busy < 0) return fail(BGPQ_EINVAL, "bad vote reset");
+    const int w[2] = {busy, 0};
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(hipMemcpyAsync(h->ctr + CTR_BUSY * kWordStride, &w[0], sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(h->ctr + CTR_TERM * kWordStride, &w[1], sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));  // the words above are on this stack
+    return BGPQ_OK;
+}
+
+int bgpq_terminated(bgpq_heap* h) {
+    if (!h) return fail(BGPQ_EINVAL, "null handle");
+    int c[CTR_WORDS];
+    const int rc = counters(h, c);
+    if (rc) return rc;
+    return c[CTR_TERM] != 0 ? 1 : 0;
+}
+
+int bgpq_clear(bgpq_heap* h, void* stream) {
+    if (!h) return fail(BGPQ_EINVAL, "null handle");
+    hipStream_t s = (hipStream_t)stream;
+    const hipError_t e =
+        h->key_type == BGPQ_KEY_U64
+            ? clear_u64(h->items, h->status, h->ctr, h->batch_num, h->batch_size, h->init_limits, s)
+            : clear_u32(h->items, h->status, h->ctr, h->batch_num, h->batch_size, (uint32_t)h->init_limits, s);
+    if (e != hipSuccess) return fail(BGPQ_EHIP, "clear: %s", hipGetErrorString(e));
+    return BGPQ_OK;
 }
 
 int bgpq_set_cursor(bgpq_heap* h, int64_t value) {

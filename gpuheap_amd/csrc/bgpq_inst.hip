@@ -42,6 +42,18 @@ hipError_t fill_u64(void* p, long long n, uint64_t v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill<KeyT>, dim3((unsigned)blocks), dim3(256), 0, s, (KeyT*)p, n, (KeyT)v);
     return hipGetLastError();
 }
+
+#if BGPQ_KEY_BITS == 32
+hipError_t clear_u32(void* items, int* status, int* ctr, int batch_num, int batch_size, uint32_t v, hipStream_t s) {
+#else
+hipError_t clear_u64(void* items, int* status, int* ctr, int batch_num, int batch_size, uint64_t v, hipStream_t s) {
+#endif
+    (void)hipGetLastError();  // report these launches only
+    hipLaunchKernelGGL(k_clear<KeyT>, dim3(1024), dim3(256), 0, s, (KeyT*)items, status, (const int*)ctr, batch_num,
+                       batch_size, (KeyT)v);
+    hipLaunchKernelGGL(k_clear_ctr<KeyT>, dim3(1), dim3(256), 0, s, ctr);
+    return hipGetLastError();
+}
 #endif
 
 }  // namespace bgpq

@@ -9,53 +9,29 @@
 
 namespace bgpq {
 
-// Workgroup size per batch size.  Every thread holds B/NT keys while sorting
-// and produces 2B/NT keys per merge.
-constexpr int threads_for(int B) {
-    return B <= 128 ? 64 : (B / 2 < 1024 ? B / 2 : 1024);
-}
-
 // gfx950 gives a workgroup up to 160 KB of LDS.  The delete path needs
 // 3*B*sizeof(K) of it: every u32 batch size up to 8192 fits, u64 up to 4096.
 constexpr int kMaxLdsBytes = 160 * 1024;
 template <typename K, int B>
 constexpr bool batch_fits() { return delete_lds_bytes<K, B>() <= kMaxLdsBytes; }
 
-// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
-// XCD b mod 8).  With pack > 1 only every pack-th workgroup of the launch
-// works, so pack = 8 keeps every worker on one XCD, whose L2 then serves the
-// coherent accesses the workers make to each other's nodes.
-struct Workers {
-    int id, count;
-    __device__ __forceinline__ static bool of(int pack, Workers& w) {
-        if ((int)blockIdx.x % pack != 0) return false;
-        w.id = (int)blockIdx.x / pack;
-        w.count = (int)gridDim.x / pack;
-        return true;
-    }
-};
-
 template <typename K, int VAR, int B>
 __global__ __launch_bounds__(threads_for(B)) void k_insert(HeapView<K> h, const K* keys, long long n,
-                                                           int chunk, int pack) {
+                                                           int chunk) {
     constexpr int NT = threads_for(B);
-    Workers w;
-    if (!Workers::of(pack, w)) return;
     DevHeap<K, VAR, B, NT> hp{h};
     const long long calls = (n + chunk - 1) / chunk;
-    for (long long i = w.id; i < calls; i += w.count) {
+    for (long long i = blockIdx.x; i < calls; i += gridDim.x) {
         const long long rem = n - i * chunk;
         hp.insertion(keys + i * chunk, (int)(rem < chunk ? rem : chunk), 0);
     }
 }
 
 template <typename K, int VAR, int B>
-__global__ __launch_bounds__(threads_for(B)) void k_delete(HeapView<K> h, K* out, long long calls, int pack) {
+__global__ __launch_bounds__(threads_for(B)) void k_delete(HeapView<K> h, K* out, long long calls) {
     constexpr int NT = threads_for(B);
-    Workers w;
-    if (!Workers::of(pack, w)) return;
     DevHeap<K, VAR, B, NT> hp{h};
-    for (long long i = w.id; i < calls; i += w.count) {
+    for (long long i = blockIdx.x; i < calls; i += gridDim.x) {
         int size = 0;
         if (hp.deleteRoot(out, size, 0)) hp.deleteUpdate(0);
     }
@@ -63,21 +39,20 @@ __global__ __launch_bounds__(threads_for(B)) void k_delete(HeapView<K> h, K* out
 
 template <typename K, int VAR, int B>
 __global__ __launch_bounds__(threads_for(B)) void k_concurrent(HeapView<K> h, const K* keys, long long n,
-                                                               int chunk, K* out, long long calls, int pack) {
+                                                               int chunk, K* out, long long calls) {
     constexpr int NT = threads_for(B);
-    Workers w;
-    if (!Workers::of(pack, w)) return;
     DevHeap<K, VAR, B, NT> hp{h};
-    const int half = w.count / 2;
-    if (w.id < half) {
+    const int half = (int)gridDim.x / 2;
+    const int id = (int)blockIdx.x;
+    if (id < half) {
         const long long ins = (n + chunk - 1) / chunk;
-        for (long long i = w.id; i < ins; i += half) {
+        for (long long i = id; i < ins; i += half) {
             const long long rem = n - i * chunk;
             hp.insertion(keys + i * chunk, (int)(rem < chunk ? rem : chunk), 0);
         }
     } else {
-        const int nd = w.count - half;
-        for (long long i = w.id - half; i < calls; i += nd) {
+        const int nd = (int)gridDim.x - half;
+        for (long long i = id - half; i < calls; i += nd) {
             int size = 0;
             if (hp.deleteRoot(out, size, 0)) hp.deleteUpdate(0);
         }
@@ -129,6 +104,25 @@ __global__ void k_fill(K* p, long long n, K v) {
         p[i] = v;
 }
 
+// bgpq_clear: every slot in use (the partial buffer and batches
+// 1..batchCount) back to `v` and their status words to 0.  Launched when no
+// kernel of the heap runs, so batchCount is read with a plain load; the
+// counters are zeroed by k_clear_ctr afterwards.
+template <typename K>
+__global__ void k_clear(K* items, int* status, const int* ctr, int batch_num, int batch_size, K v) {
+    int nb = ctr[CTR_BATCH_COUNT * kWordStride];
+    nb = nb < 0 ? 0 : (nb > batch_num ? batch_num : nb);
+    const long long n = (long long)(nb + 1) * batch_size;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (long long i = i0; i < n; i += stride) items[i] = v;
+    for (long long i = i0; i <= nb; i += stride) status[i * kWordStride] = 0;
+}
+template <typename K>
+__global__ void k_clear_ctr(int* ctr) {
+    for (int i = threadIdx.x; i < CTR_WORDS * kWordStride; i += blockDim.x) ctr[i] = 0;
+}
+
 // Type-erased launchers, one row per (key type, variant, batch size).
 struct HeapArgs {
     void* items;
@@ -138,7 +132,6 @@ struct HeapArgs {
     int batch_num;
     int heap_sort;
     long long out_cap;
-    int pack;  // workgroups per worker (Workers); 1 = every workgroup works
 };
 
 struct LaunchRow {
@@ -174,16 +167,16 @@ struct Launch {
         static const hipError_t ok = allow_lds(reinterpret_cast<const void*>(&k_insert<K, VAR, B>), lds);
         if (ok != hipSuccess) return ok;
         clear_last_error();
-        hipLaunchKernelGGL((k_insert<K, VAR, B>), dim3(grid * a.pack), dim3(threads_for(B)), lds, s, view_of<K>(a),
-                           (const K*)keys, n, chunk, a.pack);
+        hipLaunchKernelGGL((k_insert<K, VAR, B>), dim3(grid), dim3(threads_for(B)), lds, s, view_of<K>(a),
+                           (const K*)keys, n, chunk);
         return hipGetLastError();
     }
     static hipError_t del(const HeapArgs& a, void* out, long long calls, int grid, hipStream_t s) {
         static const hipError_t ok = allow_lds(reinterpret_cast<const void*>(&k_delete<K, VAR, B>), lds);
         if (ok != hipSuccess) return ok;
         clear_last_error();
-        hipLaunchKernelGGL((k_delete<K, VAR, B>), dim3(grid * a.pack), dim3(threads_for(B)), lds, s, view_of<K>(a),
-                           (K*)out, calls, a.pack);
+        hipLaunchKernelGGL((k_delete<K, VAR, B>), dim3(grid), dim3(threads_for(B)), lds, s, view_of<K>(a),
+                           (K*)out, calls);
         return hipGetLastError();
     }
     static hipError_t conc(const HeapArgs& a, const void* keys, long long n, int chunk, void* out,
@@ -191,8 +184,8 @@ struct Launch {
         static const hipError_t ok = allow_lds(reinterpret_cast<const void*>(&k_concurrent<K, VAR, B>), lds);
         if (ok != hipSuccess) return ok;
         clear_last_error();
-        hipLaunchKernelGGL((k_concurrent<K, VAR, B>), dim3(grid * a.pack), dim3(threads_for(B)), lds, s,
-                           view_of<K>(a), (const K*)keys, n, chunk, (K*)out, calls, a.pack);
+        hipLaunchKernelGGL((k_concurrent<K, VAR, B>), dim3(grid), dim3(threads_for(B)), lds, s,
+                           view_of<K>(a), (const K*)keys, n, chunk, (K*)out, calls);
         return hipGetLastError();
     }
     static hipError_t sort(const void* in, const int* sizes, void* out, uint64_t pad, int batches, hipStream_t s) {
@@ -253,5 +246,7 @@ const LaunchRow* rows_u32(int var);
 const LaunchRow* rows_u64(int var);
 hipError_t fill_u32(void* p, long long n, uint32_t v, hipStream_t s);
 hipError_t fill_u64(void* p, long long n, uint64_t v, hipStream_t s);
+hipError_t clear_u32(void* items, int* status, int* ctr, int batch_num, int batch_size, uint32_t v, hipStream_t s);
+hipError_t clear_u64(void* items, int* status, int* ctr, int batch_num, int batch_size, uint64_t v, hipStream_t s);
 
 }  // namespace bgpq

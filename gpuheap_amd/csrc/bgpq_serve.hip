@@ -63,8 +63,10 @@ struct ServeRing {
     const long long* tail;  // member slots posted
     const int* closed;
     Completion* done;       // per query index
-    long long* claim;       // device: next member slot to take
-    int* fault;             // device: a workgroup gave up waiting
+    long long* claim;       // device: member slots taken (only posted ones are)
+    int* fault;             // set when a workgroup left after max_polls idle polls
+    int* live;              // workgroups still in the kernel
+    long long max_polls;
     unsigned qmask, smask;
 };
 
@@ -133,25 +135,33 @@ __device__ void finish(const QueryDesc& d, Completion* c) {
         const int heap_err = a_load(d.ctr + CTR_ERROR * kWordStride);
         const long long left = ctl_load(d.ctl, APP_OUTSTANDING);
         int err = (int)ctl_load(d.ctl, APP_ERROR) | (heap_err << 8);
-        if (left != 0) err |= APPERR_HEAP;
+        // A* ends by the termination vote with its pruned keys in the heap:
+        // exactly the outstanding ones (the block is refilled before reuse)
+        long long in_heap = 0;
+        if (d.type == BGPQ_Q_ASTAR)
+            in_heap = (long long)a_load(d.ctr + CTR_BATCH_COUNT * kWordStride) * d.B +
+                      a_load(d.ctr + CTR_PARTIAL * kWordStride);
+        if (left != in_heap) err |= APPERR_HEAP;
         auto put = [](auto* p, auto v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); };
         put(&c->value, value);
         put(&c->digest, dig);
         put(&c->popped, ctl_load(d.ctl, APP_POPPED));
         put(&c->expanded, ctl_load(d.ctl, APP_EXPANDED));
         put(&c->stale, ctl_load(d.ctl, APP_STALE));
-        put(&c->pruned, ctl_load(d.ctl, APP_PRUNED));
+        put(&c->pruned, ctl_load(d.ctl, APP_PRUNED) + in_heap);
         put(&c->pushed, ctl_load(d.ctl, APP_PUSHED));
         put(&c->error, err);
         __hip_atomic_store(&c->done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
-// Persistent serve kernel.  Each workgroup claims the next member slot,
-// waits until the host has posted it (or closed the ring), joins that
-// query's team and runs its solve; the last member to leave publishes the
-// completion.  A workgroup ends when it claims a slot past a closed ring's
-// tail, or after kMaxRingPolls unanswered polls (then `fault` is set).
+// Persistent serve kernel.  Each workgroup takes the next member slot once
+// the host has posted it (a CAS on the claim counter, only while it is
+// below the tail, so an idle workgroup holds no slot), joins that query's
+// team and runs its solve; the last member to leave publishes the
+// completion.  A workgroup ends when the ring is closed and every posted
+// slot is taken, or after max_polls unanswered polls (then `fault` is set);
+// either way it leaves the `live` count, which the host reads.
 __global__ __launch_bounds__(kServeNT) void k_serve(ServeRing r) {
     __shared__ long long s_slot;
     __shared__ int s_go, s_last;
@@ -160,20 +170,29 @@ __global__ __launch_bounds__(kServeNT) void k_serve(ServeRing r) {
     const int t = threadIdx.x;
     while (true) {
         if (t == 0) {
-            const long long s = __hip_atomic_fetch_add(r.claim, 1ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             int go = 0;
-            long long polls = 0;
+            long long s = -1, polls = 0;
             while (true) {
-                if (s < sys_load(r.tail)) {
-                    go = 1;
-                    break;
+                const long long c = __hip_atomic_load(r.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (c < sys_load(r.tail)) {
+                    long long e = c;
+                    if (__hip_atomic_compare_exchange_strong(r.claim, &e, c + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) {
+                        s = c;
+                        go = 1;
+                        break;
+                    }
+                    continue;  // another workgroup took slot c
                 }
+                // the host posts every slot before it closes the ring, and
+                // this load of the tail issues after the closed flag is seen
                 if (sys_load(r.closed)) {
-                    go = s < sys_load(r.tail);
+                    if (__hip_atomic_load(r.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < sys_load(r.tail))
+                        continue;
                     break;
                 }
-                if (++polls > kMaxRingPolls) {
-                    __hip_atomic_fetch_or(r.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (++polls > r.max_polls) {
+                    __hip_atomic_fetch_or(r.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(127);
@@ -182,6 +201,8 @@ __global__ __launch_bounds__(kServeNT) void k_serve(ServeRing r) {
             s_slot = s;
             if (go)
                 s_entry = __hip_atomic_load(r.slots + (s_slot & r.smask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            else
+                __hip_atomic_fetch_add(r.live, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         __syncthreads();
         if (!s_go) break;
@@ -243,9 +264,10 @@ __global__ void k_poke(FillJobs j) {
         else *(uint32_t*)j.pp[k] = (uint32_t)j.pv[k];
     }
 }
-__global__ void k_ring_reset(long long* claim, int* fault) {
-    *claim = 0;
-    *fault = 0;
+__global__ void k_ring_reset(long long* claim, int* fault, int* live, int workgroups) {
+    __hip_atomic_store(claim, 0ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(fault, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(live, workgroups, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 }  // namespace bgpq
@@ -432,6 +454,7 @@ int build_query(const bgpq_query* q, char* base, const Layout& L, hipStream_t s,
         d->free_cells = q->free_cells;
         fill(d->labels, L.n_labels, 0xFFFFFFFFull, 4);
         poke(d->labels, 0, 4);
+        poke(d->ctr + CTR_BUSY * kWordStride, 1, 4);  // the seeding member, for the termination vote
         const uint32_t dx = (uint32_t)(q->width - 1), dy = (uint32_t)(q->height - 1);
         d->first_key = (uint64_t)(10u * std::max(dx, dy) + 4u * std::min(dx, dy)) << 32;
     } else {
@@ -539,10 +562,12 @@ struct bgpq_server {
     uint32_t* slots = nullptr;
     long long* tail = nullptr;
     int* closed = nullptr;
+    int* fault = nullptr;  // written by the kernel
+    int* live_wg = nullptr;  // workgroups in the kernel, written by it
     Completion* done = nullptr;
-    // device words
+    long long max_polls = kMaxRingPolls;
+    // device word
     long long* claim = nullptr;
-    int* fault = nullptr;
     hipStream_t run = nullptr, prep = nullptr;
     bool running = false;
     long long posted = 0;
@@ -582,7 +607,7 @@ extern "C" int bgpq_server_create(bgpq_server** out, int64_t hbm_budget, int wor
     s->workgroups = default_workgroups(workgroups);
     s->qcap = qcap;
     s->scap = qcap * 16;
-    const size_t hbytes = sizeof(QueryDesc) * qcap + sizeof(Completion) * qcap + 4ull * s->scap + 256;
+    const size_t hbytes = sizeof(QueryDesc) * qcap + sizeof(Completion) * qcap + 4ull * s->scap + 512;
     hipError_t e = hipHostMalloc(&s->host, hbytes, hipHostMallocCoherent | hipHostMallocMapped);
     if (e == hipSuccess) e = hipMalloc((void**)&s->arena_base, (size_t)hbm_budget);
     if (e == hipSuccess) e = hipMalloc((void**)&s->claim, 256);
@@ -600,7 +625,8 @@ extern "C" int bgpq_server_create(bgpq_server** out, int64_t hbm_budget, int wor
     s->slots = (uint32_t*)(h + sizeof(QueryDesc) * qcap + sizeof(Completion) * qcap);
     s->tail = (long long*)(h + hbytes - 128);
     s->closed = (int*)(h + hbytes - 64);
-    s->fault = (int*)((char*)s->claim + 128);
+    s->live_wg = (int*)(h + hbytes - 192);
+    s->fault = (int*)(h + hbytes - 256);
     s->arena.init((size_t)hbm_budget);
     s->qi_busy.assign(qcap, 0);
     *out = s;
@@ -613,18 +639,36 @@ extern "C" int bgpq_server_start(bgpq_server* s) {
     __atomic_store_n(s->tail, 0ll, __ATOMIC_RELEASE);
     __atomic_store_n(s->closed, 0, __ATOMIC_RELEASE);
     s->posted = 0;
+    __atomic_store_n(s->live_wg, s->workgroups, __ATOMIC_RELEASE);
+    __atomic_store_n(s->fault, 0, __ATOMIC_RELEASE);
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_ring_reset, dim3(1), dim3(1), 0, s->run, s->claim, s->fault);
-    const ServeRing r{s->desc, s->slots, s->tail, s->closed, s->done, s->claim, s->fault, s->qcap - 1, s->scap - 1};
+    hipLaunchKernelGGL(k_ring_reset, dim3(1), dim3(1), 0, s->run, s->claim, s->fault, s->live_wg, s->workgroups);
+    const ServeRing r{s->desc, s->slots,   s->tail,      s->closed,    s->done,
+                      s->claim, s->fault, s->live_wg, s->max_polls, s->qcap - 1, s->scap - 1};
     hipLaunchKernelGGL(k_serve, dim3(s->workgroups), dim3(kServeNT), app_lds_bytes<1024>(), s->run, r);
     BGPQ_HIPCHK(hipGetLastError());
     s->running = true;
     return BGPQ_OK;
 }
 
+extern "C" int bgpq_server_set_idle_limit(bgpq_server* s, int64_t polls) {
+    if (!s || polls < 0) return bgpq_fail(BGPQ_EINVAL, "bad idle limit");
+    if (s->running) return bgpq_fail(BGPQ_EINVAL, "set the idle limit before bgpq_server_start");
+    s->max_polls = polls ? (long long)polls : kMaxRingPolls;
+    return BGPQ_OK;
+}
+
+extern "C" int bgpq_server_live(const bgpq_server* s) {
+    if (!s) return bgpq_fail(BGPQ_EINVAL, "null server");
+    return s->running ? __atomic_load_n(s->live_wg, __ATOMIC_ACQUIRE) : 0;
+}
+
 extern "C" int bgpq_server_submit(bgpq_server* s, const bgpq_query* q) {
     if (!s) return bgpq_fail(BGPQ_EINVAL, "null server");
     if (!s->running) return bgpq_fail(BGPQ_EINVAL, "server not running");
+    // no workgroup is left to take the query: refuse it rather than queue it
+    if (__atomic_load_n(s->live_wg, __ATOMIC_ACQUIRE) <= 0)
+        return bgpq_fail(BGPQ_EHEAP, "the serve kernel has no workgroup left (idle limit reached)");
     int rc = check_query(q);
     if (rc) return rc;
     if (s->live.size() >= s->qcap || s->slots_live + q->team > (long long)s->scap) return BGPQ_EAGAIN;
@@ -664,6 +708,9 @@ extern "C" int bgpq_server_submit(bgpq_server* s, const bgpq_query* q) {
 
 extern "C" int bgpq_server_poll(bgpq_server* s, bgpq_result* out, int max_results) {
     if (!s) return bgpq_fail(BGPQ_EINVAL, "null server");
+    // read before the done flags: with no workgroup left, a query still
+    // not done below can never complete
+    const bool gone = s->running && __atomic_load_n(s->live_wg, __ATOMIC_ACQUIRE) <= 0;
     int n = 0;
     for (size_t i = 0; i < s->live.size() && n < max_results;) {
         bgpq_server::Live& L = s->live[i];
@@ -686,6 +733,9 @@ extern "C" int bgpq_server_poll(bgpq_server* s, bgpq_result* out, int max_result
         s->live[i] = s->live.back();
         s->live.pop_back();
     }
+    if (n == 0 && gone && !s->live.empty())
+        return bgpq_fail(BGPQ_EHEAP, "%zu queries cannot complete: the serve kernel has no workgroup left",
+                         s->live.size());
     return n;
 }
 
@@ -700,9 +750,12 @@ extern "C" int bgpq_server_close(bgpq_server* s) {
     hipError_t e = hipStreamSynchronize(s->run);
     s->running = false;
     if (e != hipSuccess) return bgpq_fail(BGPQ_EHIP, "serve kernel: %s", hipGetErrorString(e));
-    int fault = 0;
-    BGPQ_HIPCHK(hipMemcpy(&fault, s->fault, sizeof fault, hipMemcpyDeviceToHost));
-    if (fault) return bgpq_fail(BGPQ_EHEAP, "a serve workgroup stopped waiting for the host");
+    // Workgroups that left at the idle limit are no fault; a posted query
+    // that the ended kernel never completed is.
+    size_t lost = 0;
+    for (const auto& L : s->live)
+        if (!__atomic_load_n(&s->done[L.qi].done, __ATOMIC_ACQUIRE)) ++lost;
+    if (lost) return bgpq_fail(BGPQ_EHEAP, "%zu posted queries never completed", lost);
     return BGPQ_OK;
 }
 
@@ -743,7 +796,7 @@ extern "C" int bgpq_batch_run(const bgpq_query* q, int n, int64_t hbm_budget, in
     if (qcap > (1u << 24)) return bgpq_fail(BGPQ_EINVAL, "batch too large");
     const size_t o_done = sizeof(QueryDesc) * qcap, o_slots = o_done + sizeof(Completion) * qcap;
     const size_t o_tail = align_up(o_slots + 4ull * scap, 256), o_closed = o_tail + 64, o_claim = o_tail + 128,
-                 o_fault = o_tail + 192, rbytes = o_tail + 256;
+                 o_live = o_tail + 160, o_fault = o_tail + 192, rbytes = o_tail + 256;
     char* arena = nullptr;
     char* ring = nullptr;
     hipStream_t s = nullptr;
@@ -780,14 +833,14 @@ extern "C" int bgpq_batch_run(const bgpq_query* q, int n, int64_t hbm_budget, in
             rc = bgpq_fail(BGPQ_EHIP, "begin capture");
         } else {
             for (int i = 0; i < n; ++i) launch_fills(jobs[i], s, 64);
+            const int wgs = default_workgroups(workgroups);
             hipLaunchKernelGGL(k_ring_reset, dim3(1), dim3(1), 0, s, (long long*)(ring + o_claim),
-                               (int*)(ring + o_fault));
+                               (int*)(ring + o_fault), (int*)(ring + o_live), wgs);
             const ServeRing r{(const QueryDesc*)ring, (const uint32_t*)(ring + o_slots),
                               (const long long*)(ring + o_tail), (const int*)(ring + o_closed),
                               (Completion*)(ring + o_done), (long long*)(ring + o_claim), (int*)(ring + o_fault),
-                              qcap - 1, scap - 1};
-            hipLaunchKernelGGL(k_serve, dim3(default_workgroups(workgroups)), dim3(kServeNT),
-                               app_lds_bytes<1024>(), s, r);
+                              (int*)(ring + o_live), kMaxRingPolls, qcap - 1, scap - 1};
+            hipLaunchKernelGGL(k_serve, dim3(wgs), dim3(kServeNT), app_lds_bytes<1024>(), s, r);
             if (hipStreamEndCapture(s, &graph) != hipSuccess || !graph) rc = bgpq_fail(BGPQ_EHIP, "end capture");
         }
     }

@@ -49,21 +49,3 @@ extern "C" int bgpq_gen_keys(int key_type, uint64_t seed, int64_t n, void* d_out
                            (uint64_t*)d_out);
     return hipGetLastError() == hipSuccess ? BGPQ_OK : BGPQ_EHIP;
 }
-
-namespace {
-__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, long long n16) {
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n16; i += (long long)gridDim.x * blockDim.x)
-        dst[i] = src[i];
-}
-}  // namespace
-
-// Plain device copy of `bytes` (a multiple of 16): the calibration kernel for
-// HBM byte counters (its traffic is exactly bytes read + bytes written).
-extern "C" int bgpq_dev_copy(void* d_dst, const void* d_src, int64_t bytes, void* stream) {
-    if (bytes < 0 || bytes % 16 || (bytes && (!d_dst || !d_src))) return BGPQ_EINVAL;
-    if (!bytes) return BGPQ_OK;
-    (void)hipGetLastError();  // report this launch only
-    hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, (hipStream_t)stream, (const uint4*)d_src, (uint4*)d_dst,
-                       (long long)(bytes / 16));
-    return hipGetLastError() == hipSuccess ? BGPQ_OK : BGPQ_EHIP;
-}

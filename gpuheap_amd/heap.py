@@ -160,6 +160,26 @@ class Heap:
                                   ctr.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         return items, status, ctr
 
+    def view(self):
+        """The heap's device storage (bgpq_view_t) for kernels written
+        against include/bgpq_device.hpp."""
+        v = _lib.View()
+        check(lib().bgpq_view(self._h, ctypes.byref(v)))
+        return v
+
+    def vote_reset(self, busy=0):
+        """Start a termination vote with `busy` seeding workgroups."""
+        check(lib().bgpq_vote_reset(self._h, int(busy), None))
+
+    def ifTerminate(self):
+        """The termination vote has ended (reference: ifTerminate)."""
+        return check(lib().bgpq_terminated(self._h)) == 1
+
+    def clear(self):
+        """Empty the heap, touching only the slots in use."""
+        check(lib().bgpq_clear(self._h, None))
+        sync()
+
     def printHeap(self):
         items, status, ctr = self.snapshot()
         b = self.batch_size

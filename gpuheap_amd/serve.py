@@ -107,6 +107,14 @@ class Server:
         self._h = h
         self._pending = {}
 
+    def set_idle_limit(self, polls):
+        """Idle polls before a workgroup leaves the kernel (0: default)."""
+        check(lib().bgpq_server_set_idle_limit(self._h, int(polls)))
+
+    def live(self):
+        """Workgroups of the running kernel that have not left it."""
+        return check(lib().bgpq_server_live(self._h))
+
     def start(self):
         check(lib().bgpq_server_start(self._h))
         return self

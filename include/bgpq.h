@@ -87,6 +87,43 @@ int bgpq_concurrent(bgpq_heap* h, const void* d_keys, int64_t n_insert, int chun
 int bgpq_heapsort(bgpq_heap* h, const void* d_in, void* d_out, int64_t n, int chunk, int grid_insert,
                   int grid_delete, float* ms_insert, float* ms_delete, void* stream);
 
+/* The heap's device storage, for kernels written outside this library
+ * against include/bgpq_device.hpp (bgpq::view_from<K>(v) builds the
+ * DevHeap's HeapView from it).  This replaces the reference's shallow
+ * device copy of the Heap<K> object (test/heap_test.cu:128-129): the
+ * pointers stay owned by the handle.  Status words and counters sit
+ * word_stride ints apart. */
+typedef struct bgpq_view_t {
+    void* items;     /* (batch_num + 1) * batch_size keys */
+    int* status;     /* (batch_num + 1) * word_stride ints */
+    int* ctr;        /* 8 * word_stride ints */
+    uint64_t init_limits;
+    int32_t batch_num;
+    int32_t batch_size;
+    int32_t variant;   /* BGPQ_BUITDD .. BGPQ_TDITDD_NP */
+    int32_t key_type;  /* BGPQ_KEY_U32 / BGPQ_KEY_U64 */
+    int32_t word_stride;
+    int32_t reserved;
+} bgpq_view_t;
+int bgpq_view(bgpq_heap* h, bgpq_view_t* out);
+
+/* Termination vote of persistent kernels (the reference's PBS_MODEL
+ * tbstate[] / terminate, heap/tditdd/heap.cuh:282-289 and
+ * artifacts/astar/BGPQ_B/heap.cuh:273-283): a workgroup is busy from a
+ * deleteRoot that hands it keys until a deleteRoot finds nothing for it;
+ * the vote ends when no workgroup is busy.  bgpq_vote_reset starts a vote
+ * with `busy` workgroups counted busy (those that call voteBusy() because
+ * they seed the heap, as the reference presets tbstate[0] = 1).
+ * bgpq_terminated returns 1 once the vote has ended, 0 before. */
+int bgpq_vote_reset(bgpq_heap* h, int busy, void* stream);
+int bgpq_terminated(bgpq_heap* h);
+
+/* Empty the heap like bgpq_reset, touching only the slots in use (the
+ * partial buffer and batches 1..batchCount, their status words, and the
+ * counters): for a heap that a threshold search left holding keys it
+ * will never pop.  Stream-ordered. */
+int bgpq_clear(bgpq_heap* h, void* stream);
+
 /* Host copies of the whole device state, for parity tests: items
  * ((batch_num+1)*batch_size keys), status (batch_num+1 ints), counters (8 ints:
  * batchCount, partialBufferSize, deleteCount, error flags).  Any may be NULL. */
@@ -119,10 +156,6 @@ int bgpq_test_merge(int key_type, int batch_size, const void* d_a, const void* d
  * SplitMix64 output for `seed`; u32 keys lie in [0, 2^32-2], u64 keys are
  * (u32 key << 32) | i.  Ordered on `stream`. */
 int bgpq_gen_keys(int key_type, uint64_t seed, int64_t n, void* d_out, void* stream);
-
-/* Plain device-to-device copy kernel (bytes a multiple of 16); used to
- * calibrate HBM byte counters against a kernel of known traffic. */
-int bgpq_dev_copy(void* d_dst, const void* d_src, int64_t bytes, void* stream);
 
 const char* bgpq_last_error(void);
 

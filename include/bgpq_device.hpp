@@ -3,9 +3,11 @@
 // This is the workgroup-collective interface that the reference exposes as
 // the Heap<K> class template (heap/{buitdd,buitdd_np,tditdd,tditdd_np}/heap.cuh):
 //   bool deleteRoot(K* items, int& size)               (buitdd:283-343, tditdd:267-341)
+//   bool deleteRoot(K* items, int& size, K threshold)  (artifacts/astar/BGPQ_B/heap.cuh:260-329)
 //   void deleteUpdate(int smemOffset)                   (buitdd:348-482, tditdd:345-534)
 //   void insertion(const K* items, int size, int smemOffset)   (buitdd:484-690, tditdd:536-753)
 //   int  getItemCount()                                 (buitdd:206-211)
+//   bool ifTerminate()                                  (tditdd:117-120, PBS_MODEL)
 // Every thread of the workgroup must call them.  deleteUpdate must follow a
 // deleteRoot that returned true, on the same DevHeap object (it carries the
 // state deleteRoot found under status[0]: the batch taken early, the partial
@@ -15,11 +17,17 @@
 // insertion 2*B*sizeof(K) (3*B*sizeof(K) for B >= 2048), deleteUpdate
 // 3*B*sizeof(K).
 //
+// A kernel written outside this library gets the heap's storage from the
+// host handle with bgpq_view() (include/bgpq.h) and builds its DevHeap with
+// view_from<K>(): the counterpart of the reference's shallow device copy of
+// the Heap<K> object (test/heap_test.cu:128-129).  DevHeap's VAR and B must
+// match the view's variant and batch_size.
+//
 // Storage (HBM): items[(batch_num+1)*B]: slot 0 is the partial buffer,
 // slots 1..batchCount the full sorted batches, every other slot holds
 // init_limits.  One status word per slot and the counters (batchCount,
 // partialBufferSize, deleteCount = heap-sort output cursor, error flags,
-// abort word) each sit alone in a 128-byte line
+// abort word, the termination vote) each sit alone in a 128-byte line
 // (kWordStride ints apart), so spinning workgroups do not share a line
 // with the words the lock holder updates.
 //
@@ -31,6 +39,7 @@
 // bu variants address slot s as logical tree node rev(s) (bit reversal below
 // the leading one), so consecutive new slots take disjoint insert paths.
 #pragma once
+#include "bgpq.h"
 #include "bgpq_prims.hpp"
 
 // Phase hook for timeline tracing builds: such a build defines
@@ -42,7 +51,19 @@
 namespace bgpq {
 
 enum { VAR_BUITDD = 0, VAR_BUITDD_NP = 1, VAR_TDITDD = 2, VAR_TDITDD_NP = 3 };
-enum { CTR_BATCH_COUNT = 0, CTR_PARTIAL = 1, CTR_DELETE_COUNT = 2, CTR_ERROR = 3, CTR_ABORT = 4, CTR_WORDS = 8 };
+// Counter words.  CTR_BUSY and CTR_TERM are the termination vote of the
+// reference's PBS_MODEL (tbstate[] and terminate, tditdd:282-289), kept as
+// a count of busy workgroups instead of one flag per workgroup.
+enum {
+    CTR_BATCH_COUNT = 0,
+    CTR_PARTIAL = 1,
+    CTR_DELETE_COUNT = 2,
+    CTR_ERROR = 3,
+    CTR_ABORT = 4,
+    CTR_BUSY = 5,
+    CTR_TERM = 6,
+    CTR_WORDS = 8
+};
 enum { ERR_OVERFLOW = 1, ERR_BAD_SIZE = 2, ERR_LOST_KEYS = 4, ERR_SPIN_LIMIT = 8, ERR_OUTPUT_FULL = 16 };
 
 // Ints between consecutive status words / counters: one 128-byte line each.
@@ -66,6 +87,17 @@ struct HeapView {
     int heap_sort;     // nonzero: deleteRoot writes at items + deleteCount (HEAP_SORT mode)
     long long out_cap; // heap-sort mode: keys the caller's output buffer holds
 };
+
+// The device view of a heap described by bgpq_view() on the host.  Popped
+// batches go to the caller's buffer (no heap-sort cursor).
+template <typename K>
+__host__ __device__ inline HeapView<K> view_from(const bgpq_view_t& v) {
+    return HeapView<K>{reinterpret_cast<K*>(v.items), v.status, v.ctr, (K)v.init_limits, v.batch_num, 0, 0};
+}
+
+// Workgroup size the library's own kernels use for a batch size: every
+// thread holds B/NT keys while sorting and produces 2B/NT keys per merge.
+constexpr int threads_for(int B) { return B <= 128 ? 64 : (B / 2 < 1024 ? B / 2 : 1024); }
 
 // Slot <-> logical node map of the bu variants.  rev(1) is defined as 1: the
 // reference's device form (buitdd:233-236) shifts by 32 there, its host form
@@ -132,6 +164,9 @@ struct DevHeap {
     // (still held by deleteUpdate).  With an empty partial buffer (all
     // init_limits) the fold into the candidate is the identity and is skipped.
     mutable int pbs_ = 0;
+    // Lane 0: this workgroup's vote, busy since its last pop (counted in
+    // CTR_BUSY) until a deleteRoot finds nothing to hand out.
+    mutable int busy_ = 0;
 
     __device__ __forceinline__ K* node(int i) const { return h.items + (size_t)i * B; }
     __device__ __forceinline__ int* st(int i) const { return h.status + (size_t)i * kWordStride; }
@@ -273,11 +308,63 @@ struct DevHeap {
     // is locked, so roots are taken in heap-sort cursor order.  deleteUpdate
     // must then be called with the same smemOffset.
     __device__ bool deleteRoot(K* items, int& size, int smemOffset = -1) const {
+        return delete_root(items, size, smemOffset, false, K(0));
+    }
+    // The A* forks' deleteRoot(K*, uint32_t&, uint32_t app_t_val)
+    // (artifacts/astar/BGPQ_B/heap.cuh:260-287, BGPQ_T/heap.cuh:275-289):
+    // when the heap holds full batches and the root's smallest key is not
+    // below `threshold`, nothing is handed out (size 0, false) and the
+    // workgroup votes idle, exactly as for an empty heap.  Pass the threshold
+    // as a K (an int third argument is smemOffset).  The batch is not taken
+    // early in this form: the root is checked under its lock first.
+    __device__ bool deleteRoot(K* items, int& size, K threshold, int smemOffset = -1) const {
+        return delete_root(items, size, smemOffset, true, threshold);
+    }
+
+    // ---- termination vote (PBS_MODEL: tbstate[] / terminate, tditdd:282-289).
+    // A workgroup is busy from the pop that hands it keys until a deleteRoot
+    // finds nothing for it (the heap empty, or the root at or above the
+    // threshold); that deleteRoot, under status[0], ends the vote when no
+    // workgroup is busy any more.  Workgroups that hold keys before their
+    // first deleteRoot (the ones that seed the heap) call voteBusy() first,
+    // and the host counts them in bgpq_vote_reset().
+    __device__ void voteBusy() const {
+        if (threadIdx.x == 0) busy_ = 1;
+    }
+    // Collective: the vote has ended (reference: ifTerminate, tditdd:117-120).
+    __device__ bool ifTerminate() const {
+        int* bc = bcast_slots();
+        __syncthreads();
+        if (threadIdx.x == 0) bc[BC_F] = a_load(ctr(CTR_TERM));
+        __syncthreads();
+        const bool t = bc[BC_F] != 0;
+        __syncthreads();
+        return t;
+    }
+    // Lane 0, under status[0]: record this workgroup's vote after a
+    // deleteRoot that handed out keys (got) or nothing.
+    __device__ __forceinline__ void vote(bool got) const {
+        if (got) {
+            if (!busy_) {
+                busy_ = 1;
+                a_add(ctr(CTR_BUSY), 1);
+            }
+            return;
+        }
+        int now = a_load(ctr(CTR_BUSY));
+        if (busy_) {
+            busy_ = 0;
+            now = a_add(ctr(CTR_BUSY), -1) - 1;
+        }
+        if (now <= 0) a_store(ctr(CTR_TERM), 1);
+    }
+
+    __device__ bool delete_root(K* items, int& size, int smemOffset, bool use_thr, K thr) const {
         int* bc = bcast_slots();
         size = 0;
         pre_last_ = 0;
         if (!op_begin()) return false;
-        const bool prefetch = BU && smemOffset >= 0;
+        const bool prefetch = BU && smemOffset >= 0 && !use_thr;
         BGPQ_EVENT(1);
         if (threadIdx.x == 0) {
             do {
@@ -298,6 +385,20 @@ struct DevHeap {
                                           __HIP_MEMORY_SCOPE_AGENT);
                     mode = 0;
                 }
+                // threshold form: the root is read under its own lock (the
+                // deleter ahead may still be writing it), and a root at or
+                // above the threshold is left in place
+                bool root_held = false;
+                if (mode == 2 && use_thr) {
+                    if (!lock_cas_eager(1, AVAIL, INUSE)) break;
+                    if (!(gload_key(node(1)) < thr)) {
+                        release_st(1, AVAIL);
+                        mode = 0;
+                    } else {
+                        root_held = true;
+                    }
+                }
+                vote(mode != 0);
                 // Only a batch below the root's children may leave early (the
                 // previous deleter may still be refilling the root from slots
                 // 2 and 3).  The batch taken usually holds keys >= everything
@@ -314,7 +415,7 @@ struct DevHeap {
                     if (lock_max(last) < 0) break;
                     BGPQ_EVENT(3);
                 } else if (mode == 2) {
-                    if (!lock_cas_eager(1, AVAIL, INUSE)) break;  // one waiter, as below
+                    if (!root_held && !lock_cas_eager(1, AVAIL, INUSE)) break;  // one waiter, as below
                     if (h.heap_sort) a_store(ctr(CTR_DELETE_COUNT), off + B);
                 }
                 bc[BC_A] = mode;
@@ -901,6 +1002,16 @@ struct DevHeap {
         const int nb = bc[BC_A], pbs = bc[BC_B];
         const int mine = BU ? bc[BC_TAG] : 0;  // this insertion's INSHOLD word
         __syncthreads();
+        // A batch that goes into the tree (case 2.2, or any insertion
+        // without a partial buffer) needs slot nb + 1.  On overflow nothing
+        // is touched, the partial buffer included: status[0] is released
+        // and the error flagged.
+        const bool to_tree = !(PB && (size + pbs < B || (nb == 0 && size < B)));
+        if (to_tree && nb + 1 > h.batch_num) {
+            if (threadIdx.x == 0) release_st(0, AVAIL);
+            flag_error(ERR_OVERFLOW);
+            return;
+        }
 
         // ---- partial-buffer cases 1.1-2.2 (buitdd:522-613, tditdd:571-663).
         // The partial buffer and every counter only change under status[0],

@@ -84,20 +84,32 @@ int64_t bgpq_query_bytes(const bgpq_query* q);
  * memory before bgpq_server_start: the kernel reads them with plain loads. */
 int bgpq_server_create(bgpq_server** out, int64_t hbm_budget, int workgroups, int queue_slots);
 void bgpq_server_destroy(bgpq_server* s);
+/* Idle polls (each about 3.4 us) a workgroup of the persistent kernel makes
+ * while nothing is posted before it leaves the kernel; 0 restores the
+ * default (about a minute).  Set before bgpq_server_start. */
+int bgpq_server_set_idle_limit(bgpq_server* s, int64_t polls);
 /* Launch the persistent kernel (on its own stream). */
 int bgpq_server_start(bgpq_server* s);
+/* Workgroups of the running kernel that have not left it. */
+int bgpq_server_live(const bgpq_server* s);
 /* Prepare the query's state in the arena and post it.  Returns BGPQ_OK,
- * BGPQ_EAGAIN when the arena or the queue is full now, or an error. */
+ * BGPQ_EAGAIN when the arena or the queue is full now, or an error
+ * (BGPQ_EHEAP once every workgroup has left the kernel at the idle limit:
+ * nothing could take the query).  A workgroup takes a member slot only
+ * once it is posted, so a posted query is solved while any workgroup is
+ * left. */
 int bgpq_server_submit(bgpq_server* s, const bgpq_query* q);
 /* Collect up to max_results completed queries (their arena memory is
- * freed); returns how many were written. */
+ * freed); returns how many were written, or BGPQ_EHEAP when none completed
+ * and the queries still in flight never can (no workgroup is left). */
 int bgpq_server_poll(bgpq_server* s, bgpq_result* out, int max_results);
 /* Queries submitted and not yet collected. */
 int bgpq_server_inflight(const bgpq_server* s);
 /* Arena bytes in use. */
 int64_t bgpq_server_used_bytes(const bgpq_server* s);
 /* No more submissions: the kernel ends once every posted query is done.
- * Waits for it; collect the last results with bgpq_server_poll. */
+ * Waits for it; collect the last results with bgpq_server_poll.  Fails
+ * with BGPQ_EHEAP if a posted query never completed. */
 int bgpq_server_close(bgpq_server* s);
 
 /* Offline batch: every query of the list at once (they must all fit in

@@ -13,7 +13,9 @@
  *   heap/heaputil.cuh:99-158   imergePath    -> merge2 (ties take a first; unobservable for scalar keys)
  *   heap/buitdd/heap.cuh:233-255  getReversedIdx / hostGetReversedIdx -> rev_idx
  *   heap/buitdd/heap.cuh:283-343  deleteRoot   (tditdd:267-341, *_np without the partial-only branch)
- *   heap/buitdd/heap.cuh:348-482  deleteUpdate (buitdd_np:288-423 drops the partial merge and stops on <=)
+ *   heap/buitdd/heap.cuh:348-482  deleteUpdate (buitdd_np:288-423 drops the partial merge and stops on <=);
+ *                                 split into orc_take_last / orc_root_out / orc_sift_step so that tests can
+ *                                 script interleavings of concurrent deleters
  *   heap/tditdd/heap.cuh:345-534  deleteUpdate (natural indexing, stops on <=)
  *   heap/buitdd/heap.cuh:484-690  insertion, bottom-up (partial cases 1.1-2.2 at :522-613)
  *   heap/tditdd/heap.cuh:536-753  insertion, top-down toward the target slot
@@ -129,6 +131,15 @@ int orc_insert(orc_heap* m, const uint64_t* keys, int size) {
     for (int i = 0; i < B; ++i) X0[i] = i < size ? keys[i] : m->init;
     qsort(X0, (size_t)B, sizeof(uint64_t), cmp_u64);
     const int nb = m->batch_count, pbs = m->partial;
+    /* A batch that goes into the tree (case 2.2, or any insertion without a
+     * partial buffer) needs slot nb + 1.  The reference does not check (it
+     * writes past its array); here an overflow changes nothing and sets
+     * error 1, as the GPU heap does. */
+    const int to_tree = !(has_pb(m->variant) && (size + pbs < B || (nb == 0 && size < B)));
+    if (to_tree && nb + 1 > m->batch_num) {
+        m->error |= 1;
+        return -1;
+    }
     if (has_pb(m->variant)) {
         if (nb == 0 && size < B) {
             if (pbs == 0) { /* case 1.1 */
@@ -161,10 +172,6 @@ int orc_insert(orc_heap* m, const uint64_t* keys, int size) {
         m->partial = pbs + size - B;
     }
     const int idx = ++m->batch_count;
-    if (idx > m->batch_num) {
-        m->error |= 1;
-        return -1;
-    }
     if (is_bu(m->variant)) {
         memcpy(node(m, idx), X0, sizeof(uint64_t) * B);
         int cur = idx;
@@ -217,43 +224,82 @@ int orc_delete_root(orc_heap* m, uint64_t* out, int* size) {
     return 1;
 }
 
-void orc_delete_update(orc_heap* m) {
+/* One level step of the top-down sift (buitdd:405-480, tditdd:419-531): the
+ * deleter holds node cur and carries candidate cand.  The two children are
+ * merged, the larger half goes to the child with the larger maximum, and
+ * the smaller half competes with the candidate.  Returns the node the sift
+ * continues at, or 0 when it ends here. */
+int orc_sift_step(orc_heap* m, int cur, uint64_t* cand) {
     const int B = m->B;
-    uint64_t* X0 = m->x0;
+    uint64_t* X0 = cand;
     uint64_t* X1 = m->x1;
     uint64_t* X2 = m->x2;
-    const int last = m->batch_count--;
-    if (last == 1) return;
-    memcpy(X0, node(m, last), sizeof(uint64_t) * B);
+    const int l = left_child(m, cur), r = right_child(m, cur);
+    const int lin = l <= m->batch_num, rin = r <= m->batch_num;
+    if (lin) memcpy(X1, node(m, l), sizeof(uint64_t) * B);
+    else fill(X1, B, m->init);
+    if (rin) memcpy(X2, node(m, r), sizeof(uint64_t) * B);
+    else fill(X2, B, m->init);
+    const int tr = X1[B - 1] < X2[B - 1];
+    const int tgt = tr ? r : l, oth = tr ? l : r;
+    const int tgt_in = tr ? rin : lin, oth_in = tr ? lin : rin;
+    merge2(m, X1, X2, X1, tgt_in ? node(m, tgt) : NULL);
+    const int stop = (m->variant == ORC_BUITDD) ? (X0[B - 1] < X1[0]) : (X0[B - 1] <= X1[0]);
+    if (X0[0] >= X1[B - 1]) {
+        memcpy(node(m, cur), X1, sizeof(uint64_t) * B);
+    } else if (stop) {
+        memcpy(node(m, cur), X0, sizeof(uint64_t) * B);
+        if (oth_in) memcpy(node(m, oth), X1, sizeof(uint64_t) * B);
+        return 0;
+    } else {
+        merge2(m, X0, X1, node(m, cur), X0);
+    }
+    if (!oth_in) {
+        m->error |= 4;
+        return 0;
+    }
+    return oth;
+}
+
+/* The last batch leaves as the deleter's candidate, with the partial
+ * buffer folded in (its larger half stays there): buitdd:357-395.  Returns
+ * the slot taken, 0 when the heap holds no full batch. */
+int orc_take_last(orc_heap* m, uint64_t* cand) {
+    const int B = m->B;
+    const int last = m->batch_count;
+    if (last < 1) return 0;
+    memcpy(cand, node(m, last), sizeof(uint64_t) * B);
     fill(node(m, last), B, m->init);
-    if (has_pb(m->variant)) merge2(m, X0, node(m, 0), X0, node(m, 0));
+    m->batch_count--;
+    if (has_pb(m->variant)) merge2(m, cand, node(m, 0), cand, node(m, 0));
+    return last;
+}
+
+/* The root batch leaves at the heap-sort cursor.  With a candidate taken
+ * before the root (the GPU deleteRoot's early take of the last batch), the
+ * root check hands out the B smallest of root and candidate instead, and
+ * the candidate keeps the B largest; `check` = 0 leaves that check out, to
+ * show what it repairs.  Returns 1 when the check merged. */
+int orc_root_out(orc_heap* m, uint64_t* out, uint64_t* cand, int check) {
+    const int B = m->B, off = m->delete_count;
+    memcpy(out + off, node(m, 1), sizeof(uint64_t) * B);
+    m->delete_count += B;
+    if (cand && check && cand[0] < node(m, 1)[B - 1]) {
+        merge2(m, cand, node(m, 1), out + off, cand);
+        return 1;
+    }
+    return 0;
+}
+
+void orc_delete_update(orc_heap* m) {
+    uint64_t* X0 = m->x0;
+    if (m->batch_count == 1) {
+        m->batch_count = 0;
+        return;
+    }
+    orc_take_last(m, X0);
     int cur = 1;
-    while (1) {
-        const int l = left_child(m, cur), r = right_child(m, cur);
-        const int lin = l <= m->batch_num, rin = r <= m->batch_num;
-        if (lin) memcpy(X1, node(m, l), sizeof(uint64_t) * B);
-        else fill(X1, B, m->init);
-        if (rin) memcpy(X2, node(m, r), sizeof(uint64_t) * B);
-        else fill(X2, B, m->init);
-        const int tr = X1[B - 1] < X2[B - 1];
-        const int tgt = tr ? r : l, oth = tr ? l : r;
-        const int tgt_in = tr ? rin : lin, oth_in = tr ? lin : rin;
-        merge2(m, X1, X2, X1, tgt_in ? node(m, tgt) : NULL);
-        const int stop = (m->variant == ORC_BUITDD) ? (X0[B - 1] < X1[0]) : (X0[B - 1] <= X1[0]);
-        if (X0[0] >= X1[B - 1]) {
-            memcpy(node(m, cur), X1, sizeof(uint64_t) * B);
-        } else if (stop) {
-            memcpy(node(m, cur), X0, sizeof(uint64_t) * B);
-            if (oth_in) memcpy(node(m, oth), X1, sizeof(uint64_t) * B);
-            return;
-        } else {
-            merge2(m, X0, X1, node(m, cur), X0);
-        }
-        if (!oth_in) {
-            m->error |= 4;
-            return;
-        }
-        cur = oth;
+    while ((cur = orc_sift_step(m, cur, X0)) > 0) {
     }
 }
 

@@ -27,6 +27,9 @@ void orc_destroy(orc_heap* m);
 int orc_insert(orc_heap* m, const uint64_t* keys, int size);
 int orc_delete_root(orc_heap* m, uint64_t* out, int* size);
 void orc_delete_update(orc_heap* m);
+int orc_take_last(orc_heap* m, uint64_t* cand);
+int orc_root_out(orc_heap* m, uint64_t* out, uint64_t* cand, int check);
+int orc_sift_step(orc_heap* m, int cur, uint64_t* cand);
 int orc_check(const orc_heap* m, char* why, int why_len);
 int orc_heapsort(int variant, int B, int batch_num, uint64_t init_limits, const uint64_t* keys, int64_t n,
                  int chunk, uint64_t* out);

@@ -52,6 +52,12 @@ def lib():
         L.orc_delete_root.restype = ctypes.c_int
         L.orc_delete_root.argtypes = [P, _u64p, ctypes.POINTER(ctypes.c_int)]
         L.orc_delete_update.argtypes = [P]
+        L.orc_take_last.restype = ctypes.c_int
+        L.orc_take_last.argtypes = [P, _u64p]
+        L.orc_root_out.restype = ctypes.c_int
+        L.orc_root_out.argtypes = [P, _u64p, _u64p, ctypes.c_int]
+        L.orc_sift_step.restype = ctypes.c_int
+        L.orc_sift_step.argtypes = [P, ctypes.c_int, _u64p]
         L.orc_check.restype = ctypes.c_int
         L.orc_check.argtypes = [P, ctypes.c_char_p, ctypes.c_int]
         L.orc_heapsort.restype = ctypes.c_int
@@ -120,6 +126,28 @@ class Model:
         if more:
             lib().orc_delete_update(self._m)
         return size.value
+
+    # ---- one deletion in the steps a concurrent deleter takes, for scripted
+    # interleavings (orc_take_last / orc_root_out / orc_sift_step)
+    def take_last(self):
+        """The last batch as a new candidate (numpy array), or None."""
+        cand = np.zeros(self.B, dtype=np.uint64)
+        return cand if lib().orc_take_last(self._m, _p64(cand)) else None
+
+    def root_out(self, cand=None, check=True):
+        """The root leaves at the cursor; with a candidate taken early, the
+        root check merges them.  Returns True when it merged."""
+        self.ensure_out(self.st.delete_count + self.B)
+        c = _p64(cand) if cand is not None else None
+        return lib().orc_root_out(self._m, _p64(self.out), c, 1 if check else 0) == 1
+
+    def sift_step(self, cur, cand):
+        """One level step at node cur; the next node, or 0 when done."""
+        return lib().orc_sift_step(self._m, cur, _p64(cand))
+
+    def set_items(self, slot, keys):
+        items = np.ctypeslib.as_array(self.st.items, shape=((self.batch_num + 1) * self.B,))
+        items[slot * self.B:(slot + 1) * self.B] = np.asarray(keys, dtype=np.uint64)
 
     def check(self):
         buf = ctypes.create_string_buffer(256)

@@ -57,3 +57,23 @@ def test_cpp_facade_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", f"-I{INCLUDE}", str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_device_api_is_public():
+    # the workgroup-collective heap and its primitives are headers in include/
+    for h in ("bgpq_device.hpp", "bgpq_prims.hpp"):
+        assert os.path.exists(os.path.join(INCLUDE, h)), h
+    text = open(os.path.join(INCLUDE, "bgpq_device.hpp")).read()
+    for sym in ("struct DevHeap", "view_from", "deleteRoot(K* items, int& size, K threshold", "ifTerminate",
+                "voteBusy"):
+        assert sym in text, sym
+
+
+def test_external_consumer_uses_only_public_headers():
+    # tests/ext builds its kernel from include/ and libbgpq.so alone
+    ext = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ext")
+    src = open(os.path.join(ext, "ext_knapsack.hip")).read()
+    incs = re.findall(r'#include\s+"([^"]+)"', src)
+    assert incs and all(os.path.exists(os.path.join(INCLUDE, i)) for i in incs), incs
+    mk = open(os.path.join(ext, "Makefile")).read()
+    assert "csrc" not in mk.replace("Nothing under gpuheap_amd/csrc is included", "")

@@ -134,3 +134,43 @@ def test_server_rejects_bad_queries(gpu):
             srv.submit(S.knapsack_query(w, v, cap, batch_size=2048))
         with pytest.raises(G.BgpqError, match="team"):
             srv.submit(S.knapsack_query(w, v, cap, team=0))
+
+
+def test_server_idle_limit_refuses_after_workgroups_leave(gpu):
+    """Idle workgroups take no ring slot: once all of them have left at a
+    short idle limit, the kernel has ended normally, a submission is refused
+    with an error instead of being queued for no one, and close() is clean.
+    A server whose workgroups are still there serves a query posted after a
+    long idle wait."""
+    print("[case] serve idle limit", flush=True)
+    gm = A.GridMap(A.grid_map(64, 64, 0.25, 0), 64, 64)
+    want = O.astar(A.grid_map(64, 64, 0.25, 0), 64, 64)[0]
+    srv = S.Server(1 * GiB, workgroups=4)
+    try:
+        srv.set_idle_limit(200)  # about a millisecond of polling
+        srv.start()
+        t0 = time.time()
+        while srv.live() > 0 and time.time() - t0 < 30:
+            time.sleep(0.01)
+        assert srv.live() == 0
+        with pytest.raises(Exception, match="no workgroup left"):
+            srv.submit(S.astar_query(gm, batch_size=256, team=1, tag=1))
+        srv.close()
+    finally:
+        srv.destroy()
+    # the default limit: workgroups stay, and a query posted late is served
+    srv = S.Server(1 * GiB, workgroups=4)
+    try:
+        srv.start()
+        time.sleep(0.5)
+        assert srv.live() == 4
+        assert srv.submit(S.astar_query(gm, batch_size=256, team=2, tag=2))
+        got = []
+        t0 = time.time()
+        while not got and time.time() - t0 < 60:
+            got = srv.poll()
+            time.sleep(0.002)
+        assert got and got[0]["status"] == 0 and got[0]["value"] == want
+        srv.close()
+    finally:
+        srv.destroy()

@@ -163,3 +163,90 @@ def test_checker_catches_planted_corruption(variant):
         assert not ok
     items[child * B] = saved
     assert m.check()[0]
+
+
+def _early_take_script(check):
+    """Three deleters A, A2, C of a buitdd_np heap (B = 4), interleaved as
+    they can be on the GPU, where deleteRoot takes the last batch before it
+    waits for the root:
+
+      A  pops the root, takes slot 7 and sifts: its root step descends into
+         slot 3 (the node it then holds);
+      A2 takes slot 6, then pops the root A refilled;
+      C  takes slot 5, a child of slot 3, while A has not yet refilled
+         slot 3 from its children;
+      A  refills slot 3 without slot 5's keys, A2 refills the root;
+      C  pops the root: without the root check it would hand out a key
+         above the smallest one it carries (24 < 50).
+
+    Slots hold logical nodes rev(slot): 5 <-> 6 and 6 <-> 5.  Returns the
+    model, the concatenated output and whether C's root check merged."""
+    m = O.Model("buitdd_np", 16, 4, O.U32_MAX)
+    tree = {1: [0, 1, 2, 3], 2: [10, 11, 12, 90], 3: [20, 21, 22, 23], 4: [91, 92, 93, 94],
+            6: [95, 96, 97, 98],  # logical node 5, a child of slot 2
+            5: [24, 25, 26, 27],  # logical node 6, a child of slot 3
+            7: [50, 51, 52, 53]}
+    for slot, keys in tree.items():
+        m.set_items(slot, keys)
+    m.st.batch_count = 7
+    assert m.check()[0]
+    # A: deleteRoot + deleteUpdate's root step
+    cand_a = m.take_last()
+    m.root_out()
+    cur_a = m.sift_step(1, cand_a)
+    assert cur_a == 3
+    # A2: early take, then the root
+    cand_a2 = m.take_last()
+    assert not m.root_out(cand_a2, check)
+    # C: early take of slot 5, a child of the node A holds
+    assert O.parent("buitdd_np", m.st.batch_count) == cur_a
+    cand_c = m.take_last()
+    # A finishes at slot 3, then A2 sifts from the root
+    while cur_a:
+        cur_a = m.sift_step(cur_a, cand_a)
+    cur = 1
+    while cur:
+        cur = m.sift_step(cur, cand_a2)
+    assert m.check()[0]  # consistent without slot 5's keys
+    merged = m.root_out(cand_c, check)
+    cur = 1
+    while cur:
+        cur = m.sift_step(cur, cand_c)
+    assert m.check()[0]
+    while m.delete():
+        assert m.check()[0]
+    return m, m.out[:m.st.delete_count].copy(), merged
+
+
+def test_early_take_root_check_restores_order():
+    m, out, merged = _early_take_script(check=True)
+    assert merged
+    # C handed out the B smallest of root and candidate: 24 joined 21..23
+    assert list(out[8:12]) == [21, 22, 23, 24]
+    all_keys = list(range(4)) + [10, 11, 12, 90, 20, 21, 22, 23, 91, 92, 93, 94, 95, 96, 97, 98, 24, 25, 26, 27,
+                                 50, 51, 52, 53]
+    assert list(out) == sorted(all_keys)
+    assert m.st.error == 0
+
+
+def test_early_take_without_root_check_misorders():
+    # the same interleaving with the check left out: C hands out 50 before 24
+    _, out, merged = _early_take_script(check=False)
+    assert not merged
+    assert list(out[8:12]) == [21, 22, 23, 50]
+    assert not np.all(out[1:] >= out[:-1])
+
+
+@pytest.mark.parametrize("variant", PB)
+def test_model_overflow_leaves_partial_buffer(variant):
+    # a full heap: case 2.2 needs a new slot; nothing may change
+    B = 8
+    m = O.Model(variant, 2, B, O.U32_MAX)
+    assert m.insert(np.arange(100, 100 + B, dtype=np.uint64)) == 0
+    assert m.insert(np.arange(200, 200 + B, dtype=np.uint64)) == 0
+    assert m.insert(np.arange(300, 303, dtype=np.uint64)) == 0  # partial buffer: 3 keys
+    before = m.items(), m.counters()
+    assert m.insert(np.arange(400, 400 + B, dtype=np.uint64)) == -1
+    after = m.items(), m.counters()
+    assert np.array_equal(before[0], after[0])
+    assert after[1][:3] == before[1][:3] and after[1][3] == 1
