@@ -125,18 +125,60 @@ def run_config(n, key, variant, batch, gi, gd, steps, warmup, order="random", se
             "keys_per_s": n / ((t_ins + t_del) / 1e3), "ok": ok}
 
 
-def cpu_baseline(key, sample_log2=26, pq_log2=24):
+def cpu_model():
+    """The host CPU's model name (what lscpu prints) and its usable threads."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, len(os.sched_getaffinity(0))
+
+
+PQ_LOG2 = 26  # std::priority_queue leg: the first 2^26 keys of the array
+
+
+def start_cpu_baseline(key, n):
+    """Start the CPU legs on the same keys as the GPU run (SplitMix64 seed
+    0): std::sort of the whole array, median of 5 after a warm-up, and a
+    std::priority_queue push-all / pop-all of its first 2^26 keys, median
+    of 3.  Each runs single-threaded in a process pinned to a core of its
+    own, while the GPU legs run; finish_cpu_baseline collects them."""
     exe = os.path.join(REPO, "oracle", "_build", "cpu_baseline")
     if not os.path.exists(exe):
         subprocess.run(["make", "-C", os.path.join(REPO, "oracle")], check=True, capture_output=True)
-    r = subprocess.run([exe, key, str(1 << sample_log2), "0", "1", str(1 << pq_log2)], capture_output=True,
-                       text=True, check=True)
-    j = json.loads(r.stdout.strip().splitlines()[-1])
-    return {"value": j["std_sort_keys_per_s"], "unit": "keys/s", "cores": 1, "kind": "port",
-            "sample": (f"std::sort of 2^{sample_log2} SplitMix64 seed-0 {key} keys (1 warm-up, 1 timed), "
-                       f"single thread; std::priority_queue push-all/pop-all of 2^{pq_log2}: "
-                       f"{j['pq_keys_per_s']:.3e} keys/s"),
-            "pq_keys_per_s": j["pq_keys_per_s"], "ok": j["ok"], "host_threads_available": os.cpu_count()}
+    cores = sorted(os.sched_getaffinity(0))
+    c_sort, c_pq = cores[-1], cores[-2] if len(cores) > 1 else cores[-1]
+    pq_n = min(n, 1 << PQ_LOG2)
+
+    def pinned(core):
+        return lambda: os.sched_setaffinity(0, {core})
+
+    sort_p = subprocess.Popen([exe, key, str(n), "0", "5"], stdout=subprocess.PIPE, text=True,
+                              preexec_fn=pinned(c_sort))
+    pq_p = subprocess.Popen([exe, key, str(pq_n), "0", "0", str(pq_n), "3"], stdout=subprocess.PIPE, text=True,
+                            preexec_fn=pinned(c_pq))
+    return sort_p, pq_p, n, pq_n
+
+
+def finish_cpu_baseline(legs, key, gpu_keys_per_s):
+    sort_p, pq_p, n, pq_n = legs
+    out_s, _ = sort_p.communicate()
+    out_q, _ = pq_p.communicate()
+    js = json.loads(out_s.strip().splitlines()[-1])
+    jq = json.loads(out_q.strip().splitlines()[-1])
+    model, threads = cpu_model()
+    return {"value": js["std_sort_keys_per_s"], "unit": "keys/s", "cores": 1, "kind": "port",
+            "sample": (f"std::sort of the whole {n}-key {key} array the GPU sorts (SplitMix64 seed 0), median of "
+                       f"5 timed runs after a warm-up, 1 of {threads} host threads (pinned), while the GPU legs "
+                       f"ran; std::priority_queue push-all/pop-all of its first {pq_n} keys, median of 3"),
+            "cpu_model": model, "host_threads": threads, "std_sort_s": js["std_sort_s"],
+            "pq_keys_per_s": jq["pq_keys_per_s"], "pq_n": pq_n,
+            "gpu_over_cpu": gpu_keys_per_s / js["std_sort_keys_per_s"], "ok": bool(js["ok"] and jq["ok"])}
 
 
 def _orc():
@@ -252,57 +294,75 @@ def app_bench(args):
     return 0 if ok else 1
 
 
-def mixed_bench(args):
+def mixed_run(queries, max_scale, max_side, budget_gib, rate=0.0, retune=False, table_out=None, cpu=True,
+              stratified=False, catalog=None):
     """The mixed SSSP / A* / knapsack stream: the plain one-heap-per-launch
     path, then the persistent serve kernel closed-loop (everything arrives at
-    once: solves/s) and at a Poisson rate (p50 / p99 latency)."""
+    once: solves/s) and at a Poisson rate (p50 / p99 latency).  Every served
+    result must equal the plain path's, and (cpu) the CPU oracles'."""
     from gpuheap_amd import mixed as M
     t0 = time.perf_counter()
-    cat = M.Catalog(scales=tuple(range(10, args.max_scale + 1)), sides=tuple(
-        s for s in M.ASTAR_SIDES if s <= args.max_side))
-    print(f"[bench] catalog on the device ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
-    stream = M.make_stream(cat, args.queries)
+    cat = catalog or M.Catalog(scales=tuple(range(10, max_scale + 1)),
+                               sides=tuple(x for x in M.ASTAR_SIDES if x <= max_side),
+                               grid_seeds=4, sources_per_graph=8, knap_instances=128)
+    stream = M.make_stream(cat, queries, stratified=stratified)
     print(f"[bench] catalog and a {len(stream)}-query stream ready in {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
-    budget = int(args.budget_gib * (1 << 30))
+    budget = int(budget_gib * (1 << 30))
     tpath = os.path.join(REPO, "profiles", "autotune_mixed.json")
-    if os.path.exists(tpath) and not args.retune:
+    if os.path.exists(tpath) and not retune:
         table = M.load_table(tpath)
     else:
         table = M.autotune(cat, budget, log=lambda m: print(m, file=sys.stderr, flush=True))
-        if args.retune:
-            M.save_table(table, args.table_out or tpath)
-    print(f"[bench] size-class table: {json.dumps(table)}", file=sys.stderr, flush=True)
+        if retune:
+            M.save_table(table, table_out or tpath)
     plain, plain_lat, plain_s = M.run_plain(cat, stream, log=lambda m: print(m, file=sys.stderr, flush=True))
     print(f"[bench] plain path: {len(stream) / plain_s:.1f} q/s", file=sys.stderr, flush=True)
     served, _, makespan = M.run_served(cat, stream, table, rate=0.0, budget=budget)
     qps = len(stream) / makespan
     print(f"[bench] served closed-loop: {qps:.1f} q/s", file=sys.stderr, flush=True)
-    rate = args.rate if args.rate > 0 else 0.5 * qps
+    rate = rate if rate > 0 else 0.5 * qps
     served_r, lat, _ = M.run_served(cat, stream, table, rate=rate, budget=budget)
     same = all(M.answer_of(k, key, a) == M.answer_of(k, key, b) and b["status"] == 0
                for (k, key), a, b in zip(stream, plain, served))
     same = same and all(M.answer_of(k, key, a) == M.answer_of(k, key, b) and b["status"] == 0
                         for (k, key), a, b in zip(stream, plain, served_r))
-    line = {"metric": "mixed SSSP/A*/knapsack stream: solves/s (persistent serve kernel) and p99 latency",
-            "value": qps, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-            "ms_per_step": makespan * 1e3, "higher_is_better": True, "dtype": "u64",
-            "config": {"workload": f"{len(stream)} queries: 40% SSSP on R-MAT 2^10..2^{args.max_scale}, 40% A* on "
-                                   f"256^2..{args.max_side}^2 grids (p=0.25), 20% knapsack n=200",
-                       "hbm_budget_gib": args.budget_gib, "size_class_table": table},
-            "plain_one_heap_per_launch": {"value": len(stream) / plain_s, "unit": "solves/s",
-                                          "p50_ms": float(np.percentile(plain_lat, 50) * 1e3),
-                                          "p99_ms": float(np.percentile(plain_lat, 99) * 1e3)},
-            "speedup_vs_plain": qps / (len(stream) / plain_s),
-            "served_at_rate": {"rate_qps": rate, "p50_ms": float(np.percentile(lat, 50) * 1e3),
-                               "p99_ms": float(np.percentile(lat, 99) * 1e3)},
-            "check": "served results equal the plain path's: " + ("ok" if same else "FAILED")}
-    if not args.no_cpu:
-        line["cpu_baseline"], ok = mixed_cpu_baseline(cat, stream, plain)
-        line["check"] += "; plain results equal the CPU oracle: " + ("ok" if ok else "FAILED")
+    kinds = {k: sum(1 for x, _ in stream if x == k) for k in ("sssp", "astar", "knapsack")}
+    res = {"queries": len(stream), "mix": kinds, "distinct_problems": len(set(stream)),
+           "served_solves_per_s": qps,
+           "served_at_rate": {"rate_qps": rate, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                              "p99_ms": float(np.percentile(lat, 99) * 1e3)},
+           "plain_one_heap_per_launch": {"solves_per_s": len(stream) / plain_s,
+                                         "p50_ms": float(np.percentile(plain_lat, 50) * 1e3),
+                                         "p99_ms": float(np.percentile(plain_lat, 99) * 1e3)},
+           "speedup_vs_plain": qps / (len(stream) / plain_s),
+           "workload": f"{len(stream)} queries: 40% SSSP on R-MAT 2^10..2^{max_scale}, 40% A* on "
+                       f"256^2..{max_side}^2 grids (p=0.25), 20% knapsack n=200",
+           "hbm_budget_gib": budget_gib, "size_class_table": table,
+           "check": "served results equal the plain path's: " + ("ok" if same else "FAILED")}
+    if cpu:
+        res["cpu_baseline"], ok = mixed_cpu_baseline(cat, stream, plain)
+        res["check"] += "; plain results equal the CPU oracle: " + ("ok" if ok else "FAILED")
         same = same and ok
+    res["ok"] = same
+    return res
+
+
+def mixed_bench(args):
+    """--app mixed: the mixed stream alone, as its own JSON line."""
+    r = mixed_run(args.queries, args.max_scale, args.max_side, args.budget_gib, rate=args.rate, retune=args.retune,
+                  table_out=args.table_out, cpu=not args.no_cpu)
+    line = {"metric": "mixed SSSP/A*/knapsack stream: solves/s (persistent serve kernel) and p99 latency",
+            "value": r["served_solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": 1e3 * r["queries"] / r["served_solves_per_s"], "higher_is_better": True, "dtype": "u64",
+            "config": {"workload": r["workload"], "hbm_budget_gib": r["hbm_budget_gib"],
+                       "size_class_table": r["size_class_table"]},
+            "plain_one_heap_per_launch": r["plain_one_heap_per_launch"], "speedup_vs_plain": r["speedup_vs_plain"],
+            "served_at_rate": r["served_at_rate"], "check": r["check"]}
+    if "cpu_baseline" in r:
+        line["cpu_baseline"] = r["cpu_baseline"]
     print(json.dumps(line), flush=True)
-    return 0 if same else 1
+    return 0 if r["ok"] else 1
 
 
 def mixed_cpu_baseline(cat, stream, plain):
@@ -342,6 +402,33 @@ def mixed_cpu_baseline(cat, stream, plain):
                       f"oracle/apps.c), time scaled to the {len(stream)}-query stream"}, ok
 
 
+def prims_bench(B, nb):
+    """The batch primitives alone, for counter collection under rocprofv3:
+    the group merge, the two half merges of a sift step and the low-register
+    merge path (bgpq_test_merge algo 2, 4, 5), then the pad-and-sort of the
+    insert path (bgpq_test_sort), each on nb random u32 batches, one
+    workgroup per batch; every result is checked against numpy."""
+    rng = np.random.default_rng(1)
+    a = np.sort(rng.integers(0, 2**32 - 1, size=(nb, B), dtype=np.uint32), axis=1)
+    b = np.sort(rng.integers(0, 2**32 - 1, size=(nb, B), dtype=np.uint32), axis=1)
+    da, db = DeviceBuffer.from_numpy(a), DeviceBuffer.from_numpy(b)
+    small, large = DeviceBuffer(a.nbytes), DeviceBuffer(a.nbytes)
+    ok = True
+    u = np.sort(np.concatenate([a, b], axis=1), axis=1)
+    for algo in (2, 4, 5):
+        check(lib().bgpq_test_merge(0, B, da.ptr, db.ptr, small.ptr, large.ptr, nb, algo))
+        G.sync()
+        ok = ok and np.array_equal(small.to_numpy(np.uint32).reshape(nb, B), u[:, :B])
+        ok = ok and np.array_equal(large.to_numpy(np.uint32).reshape(nb, B), u[:, B:])
+    sizes = DeviceBuffer.from_numpy(np.full(nb, B, dtype=np.int32))
+    check(lib().bgpq_test_sort(0, B, da.ptr, sizes.ptr, small.ptr, 0xFFFFFFFF, nb))
+    G.sync()
+    ok = ok and np.array_equal(small.to_numpy(np.uint32).reshape(nb, B), a)
+    print(json.dumps({"prims": {"batch_size": B, "batches": nb, "merges": [2, 4, 5], "sort": True,
+                                "check": "ok" if ok else "FAILED"}}), flush=True)
+    return 0 if ok else 1
+
+
 def traffic_for(workload):
     """HBM bytes per launch of the dominant kernel, from the committed
     rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/hbm_traffic.json)."""
@@ -364,7 +451,7 @@ def main():
     ap.add_argument("--key", choices=sorted(KEYS), default="u32")
     ap.add_argument("--variant", choices=sorted(G.VARIANTS), default="buitdd_np")
     ap.add_argument("--batch", type=int, default=None,
-                    help="batch size (heap sort: 8192; --app: sssp 2048, astar 256, knapsack 256)")
+                    help="batch size (heap sort: 8192; --app: sssp 1024, astar 256, knapsack 256)")
     ap.add_argument("--grid-insert", type=int, default=256)
     ap.add_argument("--grid-delete", type=int, default=None,
                     help="delete workgroups (heap sort: 64) or driver workgroups (--app: sssp 32, astar 4, knapsack 8)")
@@ -375,7 +462,11 @@ def main():
                     help="key sets of the sweep, key:order pairs")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a plain 1 GiB device copy (HBM counter calibration under rocprofv3)")
-    ap.add_argument("--queries", type=int, default=2000, help="mixed: queries in the stream")
+    ap.add_argument("--queries", type=int, default=2000, help="--app mixed: queries in the stream")
+    ap.add_argument("--line-queries", type=int, default=1000,
+                    help="queries of the stratified mixed stream in the default line (0: leave it out)")
+    ap.add_argument("--extra-batches", default="1024,2048",
+                    help="batch sizes also timed (1 step) in the default line ('' for none)")
     ap.add_argument("--max-scale", type=int, default=20, help="mixed: largest SSSP scale")
     ap.add_argument("--max-side", type=int, default=2048, help="mixed: largest A* grid side")
     ap.add_argument("--budget-gib", type=float, default=64.0, help="mixed: HBM budget of the serve arena")
@@ -383,6 +474,8 @@ def main():
     ap.add_argument("--retune", action="store_true", help="mixed: run the autotuner instead of the committed table")
     ap.add_argument("--table-out", default=None,
                     help="mixed --retune: where to write the new table (default: profiles/autotune_mixed.json)")
+    ap.add_argument("--prims", type=int, nargs=2, metavar=("B", "BATCHES"),
+                    help="run the batch primitives alone (merges, sort) for counter collection")
     ap.add_argument("--app", choices=["sssp", "astar", "knapsack", "mixed"],
                     help="time one driver solve instead of the heap sort (--batch, --variant, --grid-delete apply)")
     ap.add_argument("--scale", type=int, default=22, help="SSSP: log2 of the vertex count")
@@ -396,6 +489,8 @@ def main():
               "with one all-to-all, and the mixed stream would split its queries)", file=sys.stderr)
         sys.exit(2)
     print(f"[bench] device: {G.device_name()}", file=sys.stderr, flush=True)
+    if args.prims:
+        sys.exit(prims_bench(*args.prims))
     if args.app == "mixed":
         sys.exit(mixed_bench(args))
     if args.app:
@@ -403,7 +498,7 @@ def main():
         # (R-MAT scale 22, 4096^2 grid, n = 200) on one MI355X
         # measured best (batch size, workgroups) per driver: fewer workgroups
         # pop fewer keys out of order, hence less stale and repeated work
-        dflt = {"sssp": (2048, 32), "astar": (256, 4), "knapsack": (256, 8)}.get(args.app, (1024, 64))
+        dflt = {"sssp": (1024, 32), "astar": (256, 4), "knapsack": (256, 8)}.get(args.app, (1024, 64))
         args.batch = args.batch or dflt[0]
         args.grid_delete = args.grid_delete or dflt[1]
         if args.variant not in ("buitdd", "tditdd"):
@@ -435,8 +530,15 @@ def main():
 
     print(f"[bench] heap sort n={args.n} {args.key} {args.variant} B={args.batch} "
           f"grids {args.grid_insert}/{args.grid_delete}", file=sys.stderr, flush=True)
+    cpu_legs = None
+    if not args.no_cpu:
+        # the CPU legs run on cores of their own while the GPU legs below run
+        print("[bench] CPU baseline (std::sort, std::priority_queue) started", file=sys.stderr, flush=True)
+        cpu_legs = start_cpu_baseline(args.key, args.n)
+    d_in = make_input(args.n, args.key, args.order)
+    fp_in = fingerprint(d_in.to_numpy(KEYS[args.key][1], args.n))
     r = run_config(args.n, args.key, args.variant, args.batch, args.grid_insert, args.grid_delete, args.steps,
-                   args.warmup, order=args.order)
+                   args.warmup, order=args.order, d_in=d_in, fp_in=fp_in)
     if not r["ok"]:
         print("[bench] output check FAILED", file=sys.stderr, flush=True)
     s = KEYS[args.key][2]
@@ -450,6 +552,7 @@ def main():
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                 "traffic": measured.get(f"k_{dom}") if measured else None,
                 "alg_bytes_per_launch": ab[dom]}
+    ok = bool(r["ok"])
     line = {
         "metric": METRIC,
         "value": r["keys_per_s"],
@@ -471,6 +574,30 @@ def main():
         "check": "output sorted and multiset fingerprint equal to input: " + ("ok" if r["ok"] else "FAILED"),
         "roofline": roofline,
     }
+    # the configuration's own batch sizes (its sweep stops at 2048), same keys
+    extras = {}
+    for b in (int(x) for x in args.extra_batches.split(",") if x.strip()):
+        if b == args.batch:
+            continue
+        print(f"[bench] heap sort at B={b}", file=sys.stderr, flush=True)
+        e = run_config(args.n, args.key, args.variant, b, args.grid_insert, args.grid_delete, 1, 1,
+                       order=args.order, d_in=d_in, fp_in=fp_in)
+        ok = ok and bool(e["ok"])
+        ab_b = alg_bytes(args.n, b, s)
+        extras[str(b)] = {"keys_per_s": e["keys_per_s"], "ms_insert": e["ms_insert"], "ms_delete": e["ms_delete"],
+                          "hbm_frac_total": ab_b["total"] / ((e["ms_insert"] + e["ms_delete"]) / 1e3) /
+                          (HBM_PEAK_GBS * 1e9), "check": "ok" if e["ok"] else "FAILED"}
+    if extras:
+        line["other_batch_sizes"] = extras
+    d_in.free()
+    if args.line_queries > 0:
+        # the metric's second half: batched / served solves/s and p99 latency
+        print(f"[bench] mixed stream of {args.line_queries} queries", file=sys.stderr, flush=True)
+        m = mixed_run(args.line_queries, args.max_scale, args.max_side, args.budget_gib, cpu=not args.no_cpu,
+                      stratified=True)
+        ok = ok and m.pop("ok")
+        m.pop("size_class_table", None)
+        line["mixed"] = m
     if args.calibrate:
         # a plain elementwise kernel of known traffic (reads and writes 1 GiB)
         # to scale the FETCH_SIZE / WRITE_SIZE counters
@@ -481,11 +608,11 @@ def main():
         torch.mul(src, 3, out=dst)
         torch.cuda.synchronize()
         line["calibration_copy_bytes"] = nb
-    if not args.no_cpu:
-        print("[bench] CPU baseline (std::sort, std::priority_queue) ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(args.key)
+    if cpu_legs is not None:
+        print("[bench] waiting for the CPU baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = finish_cpu_baseline(cpu_legs, args.key, r["keys_per_s"])
     print(json.dumps(line), flush=True)
-    if not r["ok"]:
+    if not ok:
         sys.exit(1)
 
 

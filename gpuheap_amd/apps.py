@@ -68,13 +68,24 @@ class Graph:
         self.d_dist = DeviceBuffer(4 * self.n)
 
 
-def sssp(heap, graph, source=0, grid=0):
+def sssp(heap, graph, source=0, grid=0, fetch=True):
     """Distances from `source` (uint32, 0xFFFFFFFF = unreachable) and the
-    solve's statistics."""
+    solve's statistics; with fetch=False the distances stay on the device
+    (graph.d_dist) and None is returned in their place."""
     st = AppStats()
     check(lib().bgpq_sssp(_heap_handle(heap), graph.d_rowptr.ptr, graph.d_col.ptr, graph.d_w.ptr, graph.n,
                           source, graph.d_dist.ptr, grid, ctypes.byref(st), None))
-    return graph.d_dist.to_numpy(np.uint32, graph.n), _stats(st)
+    return (graph.d_dist.to_numpy(np.uint32, graph.n) if fetch else None), _stats(st)
+
+
+def label_digest(graph):
+    """(digest, reached) of the distances in graph.d_dist, computed on the
+    device (bgpq_label_digest); 16 bytes come back."""
+    if getattr(graph, "_digest_buf", None) is None:
+        graph._digest_buf = DeviceBuffer(16)
+    check(lib().bgpq_label_digest(graph.d_dist.ptr, graph.n, graph._digest_buf.ptr, None))
+    d = graph._digest_buf.to_numpy(np.uint64, 2)
+    return int(d[0]), int(d[1])
 
 
 class GridMap:

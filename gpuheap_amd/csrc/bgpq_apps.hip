@@ -105,6 +105,28 @@ static const AppLaunch* app_row(int variant, int B) {
 
 hipError_t fill_u32(void* p, long long n, uint32_t v, hipStream_t s);
 
+// bgpq_label_digest: per-lane partial sums, a wave reduction, one atomic per
+// wave and word (additions mod 2^64).
+__global__ void k_digest_zero(unsigned long long* out) {
+    if (threadIdx.x < 2) out[threadIdx.x] = 0;
+}
+__global__ void k_digest(const uint32_t* labels, long long n, unsigned long long* out) {
+    unsigned long long dig = 0, reach = 0;
+    for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x) {
+        const uint32_t x = labels[v];
+        dig += ((unsigned long long)x + 1) * (2 * (unsigned long long)v + 1);
+        reach += x != 0xFFFFFFFFu;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        dig += __shfl_xor(dig, o, 64);
+        reach += __shfl_xor(reach, o, 64);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(out, dig);
+        atomicAdd(out + 1, reach);
+    }
+}
+
 }  // namespace bgpq
 
 using namespace bgpq;
@@ -323,4 +345,17 @@ extern "C" int bgpq_knapsack(bgpq_heap* h, int32_t n, const uint32_t* weight, co
         return release(bgpq_fail(BGPQ_EHIP, "knapsack: result copy"));
     *best = (uint64_t)b;
     return release(BGPQ_OK);
+}
+
+extern "C" int bgpq_label_digest(const uint32_t* d_labels, int64_t n, uint64_t* d_out2, void* stream) {
+    if (n < 0 || !d_out2 || (n > 0 && !d_labels)) return bgpq_fail(BGPQ_EINVAL, "label digest: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    long long blocks = (n + 255) / 256;
+    blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_digest_zero, dim3(1), dim3(64), 0, s, (unsigned long long*)d_out2);
+    hipLaunchKernelGGL(k_digest, dim3((unsigned)blocks), dim3(256), 0, s, d_labels, (long long)n,
+                       (unsigned long long*)d_out2);
+    BGPQ_HIPCHK(hipGetLastError());
+    return BGPQ_OK;
 }

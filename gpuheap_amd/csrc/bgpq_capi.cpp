@@ -323,8 +323,10 @@ int bgpq_snapshot(bgpq_heap* h, void* h_items, int* h_status, int* h_counters) {
         if (rc) return rc;
     }
     if (h_counters) {
-        const int rc = counters(h, h_counters);
+        int c[CTR_WORDS];
+        const int rc = counters(h, c);
         if (rc) return rc;
+        for (int i = 0; i < kPublicCounters; ++i) h_counters[i] = c[i];
     }
     return BGPQ_OK;
 }

@@ -190,8 +190,9 @@ struct Launch {
     }
     static hipError_t sort(const void* in, const int* sizes, void* out, uint64_t pad, int batches, hipStream_t s) {
         clear_last_error();
-        hipLaunchKernelGGL((k_prim_sort<K, B>), dim3(batches), dim3(threads_for(B)), B * sizeof(K), s, (const K*)in,
-                           sizes, (K*)out, (K)pad);
+        constexpr int lds = sort_xch_bytes<K, B, threads_for(B)>();
+        hipLaunchKernelGGL((k_prim_sort<K, B>), dim3(batches), dim3(threads_for(B)), lds, s, (const K*)in, sizes,
+                           (K*)out, (K)pad);
         return hipGetLastError();
     }
     template <int ALGO>

@@ -78,12 +78,18 @@ class Catalog:
         return S.knapsack_query(w, v, cap, batch_size=B, team=team, tag=tag)
 
 
-def make_stream(cat, n, seed=1):
+def make_stream(cat, n, seed=1, stratified=False):
     """n (kind, key) draws: 40% SSSP (scale uniform over the catalog's
     scales, i.e. log-uniform vertices), 40% A* (side log-uniform), 20%
-    knapsack."""
+    knapsack.  stratified: exactly those shares of the n queries, in a
+    random order."""
     rng = np.random.default_rng(seed)
-    kinds = rng.choice(3, size=n, p=MIX)
+    if stratified:
+        counts = [int(round(MIX[0] * n)), int(round(MIX[1] * n))]
+        counts.append(n - sum(counts))
+        kinds = rng.permutation(np.repeat(np.arange(3), counts))
+    else:
+        kinds = rng.choice(3, size=n, p=MIX)
     out = []
     for k in kinds:
         if k == 0:
@@ -121,9 +127,11 @@ class PlainRunner:
 
     def solve(self, kind, key):
         if kind == "sssp":
+            # the labels are summarised on the device, as the served path does
             g = self.cat.graphs[key[0]]["dev"]
-            dist, st = A.sssp(self.h[kind], g, key[1], grid=self.grid[kind])
-            return {"value": int(np.count_nonzero(dist != 0xFFFFFFFF)), "digest": S.sssp_digest(dist), "stats": st}
+            _, st = A.sssp(self.h[kind], g, key[1], grid=self.grid[kind], fetch=False)
+            digest, reached = A.label_digest(g)
+            return {"value": reached, "digest": digest, "stats": st}
         if kind == "astar":
             cost, st = A.astar(self.h[kind], self.cat.grids[key]["dev"], grid=self.grid[kind])
             return {"value": cost, "digest": 0, "stats": st}

@@ -50,11 +50,13 @@ int bgpq_sssp(bgpq_heap* h, const int32_t* d_rowptr, const int32_t* d_col, const
 /* A* on an 8-connected width x height grid from cell 0 to the last cell.
  * d_free[cell] != 0 marks free cells (row-major, cell = y * width + x);
  * moves cost 10 orthogonally and 14 diagonally into any free cell, the
- * heuristic is the octile distance.  Keys are f << 32 | cell; a popped key
- * whose f is not below the best goal cost so far is pruned (the reference's
- * threshold deleteRoot, artifacts/astar/BGPQ_B/heap.cuh:260-287).
- * d_g[width*height] is scratch for the g labels; *cost receives the optimal
- * cost (UINT32_MAX if the goal is unreachable). */
+ * heuristic is the octile distance.  Keys are f << 32 | cell.  Workgroups
+ * pop only keys whose f is below the goal's label (the reference's
+ * threshold deleteRoot, artifacts/astar/BGPQ_B/heap.cuh:260-287), and the
+ * search ends by the heap's termination vote; the keys it leaves in the
+ * heap are counted as pruned, and the heap is cleared.  d_g[width*height]
+ * is scratch for the g labels; *cost receives the optimal cost
+ * (UINT32_MAX if the goal is unreachable). */
 int bgpq_astar(bgpq_heap* h, const uint8_t* d_free, int32_t width, int32_t height, uint32_t* d_g, int grid,
                uint32_t* cost, bgpq_app_stats* stats, void* stream);
 
@@ -66,6 +68,13 @@ int bgpq_astar(bgpq_heap* h, const uint8_t* d_free, int32_t width, int32_t heigh
  * Fails with BGPQ_EHEAP if the pool or the heap overflowed. */
 int bgpq_knapsack(bgpq_heap* h, int32_t n, const uint32_t* weight, const uint32_t* value, uint64_t capacity,
                   int64_t pool_nodes, int grid, uint64_t* best, bgpq_app_stats* stats, void* stream);
+
+/* Summary of SSSP labels computed on the device, the same one the serving
+ * path reports (bgpq_result.digest and .value): d_out2[0] = sum over v of
+ * (dist[v] + 1) * (2v + 1) mod 2^64, d_out2[1] = vertices reached
+ * (dist != UINT32_MAX).  d_out2 is a caller-owned device buffer of two
+ * u64; the result is stream-ordered. */
+int bgpq_label_digest(const uint32_t* d_labels, int64_t n, uint64_t* d_out2, void* stream);
 
 /* ---- host generators of the synthetic inputs (deterministic for a seed) */
 

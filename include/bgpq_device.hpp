@@ -62,20 +62,24 @@ enum {
     CTR_ABORT = 4,
     CTR_BUSY = 5,
     CTR_TERM = 6,
-    CTR_WORDS = 8
+    CTR_TICKET = 8,   // status[0]'s ticket lock: next ticket to hand out
+    CTR_SERVING = 9,  // and the ticket that holds the lock
+    CTR_WORDS = 10
 };
+// Counters a host snapshot reports (the ticket words are internal).
+constexpr int kPublicCounters = 8;
 enum { ERR_OVERFLOW = 1, ERR_BAD_SIZE = 2, ERR_LOST_KEYS = 4, ERR_SPIN_LIMIT = 8, ERR_OUTPUT_FULL = 16 };
 
 // Ints between consecutive status words / counters: one 128-byte line each.
 constexpr int kWordStride = 32;
 
 // A lock wait longer than this many polls (each at least one device-scope
-// load plus a sleep, so tens of seconds in total) trips the heap's abort
-// word: every workgroup then leaves its current operation at the next
-// hand-off point, so a protocol fault ends the kernel with ERR_SPIN_LIMIT
-// instead of hanging.  The locks are not fair: with hundreds of workgroups
-// on status[0] a waiter can lose for seconds, which must not count as a fault.
-constexpr int kSpinLimit = 1 << 25;
+// load plus a sleep, seconds in total) trips the heap's abort word: every
+// workgroup then leaves its current operation at the next hand-off point,
+// so a protocol fault ends the kernel with ERR_SPIN_LIMIT instead of
+// hanging.  status[0], which every operation takes, is a fair (FIFO) ticket
+// lock, so no waiter there can lose indefinitely to the others.
+constexpr int kSpinLimit = 1 << 22;
 
 template <typename K>
 struct HeapView {
@@ -142,6 +146,7 @@ template <typename K, int VAR, int B, int NT>
 struct DevHeap {
     static_assert((B & (B - 1)) == 0 && B >= 64, "batch size must be a power of two >= 64");
     static_assert(NT % 64 == 0 && NT <= B && NT <= 1024, "threads per workgroup");
+    static_assert(sort_xch_bytes<K, B, NT>() <= insert_lds_bytes<K, B>(), "the sort's exchange fits the workspace");
     static constexpr bool BU = (VAR == VAR_BUITDD || VAR == VAR_BUITDD_NP);
     static constexpr bool PB = (VAR == VAR_BUITDD || VAR == VAR_TDITDD);
     // state words (buitdd:6-9, tditdd:6-9).  bu words carry a tag: an
@@ -167,6 +172,8 @@ struct DevHeap {
     // Lane 0: this workgroup's vote, busy since its last pop (counted in
     // CTR_BUSY) until a deleteRoot finds nothing to hand out.
     mutable int busy_ = 0;
+    // Lane 0: the ticket this workgroup holds status[0] with.
+    mutable unsigned t0_ = 0;
 
     __device__ __forceinline__ K* node(int i) const { return h.items + (size_t)i * B; }
     __device__ __forceinline__ int* st(int i) const { return h.status + (size_t)i * kWordStride; }
@@ -258,6 +265,37 @@ struct DevHeap {
             backoff();
         }
     }
+    // status[0], the lock every operation takes first (the partial buffer,
+    // the counters and the order of roots), is a ticket lock: one atomic per
+    // acquisition, waiters served in arrival order, and each waiter polls
+    // the serving word with a sleep that grows with its place in the queue
+    // (the one next in line polls at once).  The status word itself stays
+    // AVAIL.  Lane 0; false on abort.
+    __device__ __forceinline__ bool lock0() const {
+        const unsigned t = (unsigned)a_add(ctr(CTR_TICKET), 1);
+        t0_ = t;
+        int n = 0;
+        while (true) {
+            const unsigned d = t - (unsigned)a_load(ctr(CTR_SERVING));
+            if (d == 0) return true;
+            if (poll_abort(n)) return false;
+            for (unsigned k = 1; k < d && k < 32; ++k) __builtin_amdgcn_s_sleep(8);
+            backoff();
+        }
+    }
+    // Lane 0: this lane's earlier stores (counters, the partial buffer's
+    // words) are acknowledged first, then the next ticket is served.
+    __device__ __forceinline__ void unlock0() const {
+        wait_stores();
+        a_store(ctr(CTR_SERVING), (int)(t0_ + 1));
+    }
+    // Release of the node above `cur` in a td insertion walk (status[0]
+    // above the root).
+    __device__ __forceinline__ void release_above(int cur) const {
+        if (cur >> 1) release_st(cur >> 1, AVAIL);
+        else unlock0();
+    }
+
     // Uniform abort test after a hand-off barrier.
     __device__ __forceinline__ bool aborted() const { return bcast_slots()[BC_ABORT] != 0; }
     // Start of every operation: thread 0 samples the abort word, all threads
@@ -368,7 +406,7 @@ struct DevHeap {
         BGPQ_EVENT(1);
         if (threadIdx.x == 0) {
             do {
-                if (!lock_cas(0, AVAIL, INUSE)) break;
+                if (!lock0()) break;
                 BGPQ_EVENT(2);
                 const int nb = a_load(ctr(CTR_BATCH_COUNT));
                 const int pbs = a_load(ctr(CTR_PARTIAL));
@@ -430,7 +468,7 @@ struct DevHeap {
         pbs_ = pbs;
         __syncthreads();
         if (mode == 0) {
-            if (threadIdx.x == 0) release_st(0, AVAIL);
+            if (threadIdx.x == 0) unlock0();
             return false;
         }
         if (mode == 1) {
@@ -444,7 +482,7 @@ struct DevHeap {
             if (threadIdx.x == 0) {
                 if (h.heap_sort) a_store(ctr(CTR_DELETE_COUNT), off + pbs);
                 a_store(ctr(CTR_PARTIAL), 0);
-                release_st(0, AVAIL);
+                unlock0();
             }
             size = pbs;
             return false;
@@ -471,7 +509,7 @@ struct DevHeap {
                 lock_cas_eager(1, AVAIL, INUSE);  // only the status[0] holder waits here
                 BGPQ_EVENT(5);
                 if (h.heap_sort) a_store(ctr(CTR_DELETE_COUNT), off + B);
-                release_st(0, AVAIL);
+                unlock0();
             }
             __syncthreads();
             if (aborted()) return false;
@@ -534,7 +572,7 @@ struct DevHeap {
             const int last = a_add(ctr(CTR_BATCH_COUNT), -1);
             if (last == 1) {
                 release_st(1, AVAIL);
-                release_st(0, AVAIL);
+                unlock0();
             }
             bc[BC_A] = last;
         }
@@ -592,7 +630,7 @@ struct DevHeap {
                 __syncthreads();
             }
         }
-        if (threadIdx.x == 0) release_st(0, AVAIL);
+        if (threadIdx.x == 0) unlock0();
         if constexpr (PB && kSplitFold) {
             if (fold) fold_small(X0, X1);
         }
@@ -844,17 +882,15 @@ struct DevHeap {
                 if constexpr (kAhead) {
                     if (threadIdx.x < 2) bc[BC_C + threadIdx.x] = pre;
                 }
+                // the larger child stored one level up is released at the
+                // barrier below: every wave waits for its own stores to it
+                // first, before it issues the node's (issued a whole half
+                // merge ago, they have nearly always landed by now), so the
+                // next deleter can load it while it waits for this node
+                if (pend) wait_stores();
                 // the node's new batch in the usual case (below), stored
                 // before the barrier; the rare paths overwrite it
                 if (oth_in) lo.put(ToGlobal<K>{node(cur)});
-                // the larger child stored one level up is released here, once
-                // acknowledged (its stores are older than the node's, and
-                // only stores are in flight): the next deleter can load it
-                // while it waits for this node
-                if (pend) {
-                    if (oth_in) wait_vmem_but<HalfMerge<K, B, NT, false>::kGlobalStores>();
-                    else wait_stores();
-                }
                 __syncthreads();
                 if (pend) {
                     if (threadIdx.x == 0) a_store(st(pend), pend_word);
@@ -987,7 +1023,7 @@ struct DevHeap {
         BGPQ_EVENT(21);
 
         if (threadIdx.x == 0) {
-            if (lock_cas(0, AVAIL, INUSE)) {
+            if (lock0()) {
                 BGPQ_EVENT(22);
                 bc[BC_A] = a_load(ctr(CTR_BATCH_COUNT));
                 bc[BC_B] = PB ? a_load(ctr(CTR_PARTIAL)) : 0;
@@ -1008,7 +1044,7 @@ struct DevHeap {
         // and the error flagged.
         const bool to_tree = !(PB && (size + pbs < B || (nb == 0 && size < B)));
         if (to_tree && nb + 1 > h.batch_num) {
-            if (threadIdx.x == 0) release_st(0, AVAIL);
+            if (threadIdx.x == 0) unlock0();
             flag_error(ERR_OVERFLOW);
             return;
         }
@@ -1023,7 +1059,7 @@ struct DevHeap {
                     publish();
                     if (threadIdx.x == 0) {
                         a_store(ctr(CTR_PARTIAL), size);
-                        release_st(0, AVAIL);
+                        unlock0();
                     }
                     return;
                 }
@@ -1034,7 +1070,7 @@ struct DevHeap {
                     publish();
                     if (threadIdx.x == 0) {
                         a_store(ctr(CTR_PARTIAL), pbs + size);
-                        release_st(0, AVAIL);
+                        unlock0();
                     }
                     return;
                 }
@@ -1050,7 +1086,7 @@ struct DevHeap {
                 if (threadIdx.x == 0) {
                     a_store(ctr(CTR_PARTIAL), pbs + size - B);
                     release_st(1, AVAIL);
-                    release_st(0, AVAIL);
+                    unlock0();
                 }
                 return;
             }
@@ -1073,7 +1109,7 @@ struct DevHeap {
                     if (threadIdx.x == 0) {
                         a_store(ctr(CTR_PARTIAL), pbs + size);
                         release_st(1, AVAIL);
-                        release_st(0, AVAIL);
+                        unlock0();
                     }
                     return;
                 }
@@ -1098,7 +1134,7 @@ struct DevHeap {
                     // the new slot is nearly always free: try the CAS first
                     bc[BC_C] = (a_cas(st(idx), AVAIL, INUSE) || lock_cas(idx, AVAIL, INUSE)) ? idx : -2;
                 }
-                release_st(0, AVAIL);
+                unlock0();
             }
             __syncthreads();
             if (aborted()) return;
@@ -1228,7 +1264,7 @@ struct DevHeap {
                 const int tgt = nb + 1;
                 if (tgt > h.batch_num) {
                     bc[BC_C] = -1;
-                    release_st(0, AVAIL);
+                    unlock0();
                 } else {
                     a_store(ctr(CTR_BATCH_COUNT), tgt);
                     bc[BC_C] = tgt;
@@ -1250,7 +1286,7 @@ struct DevHeap {
                     // release the node above (for the root this is status[0]);
                     // its data was published at the end of the previous step
                     const int m = (a_load(st(tgt)) == MARKED);
-                    if (!m) release_st(cur >> 1, AVAIL);
+                    if (!m) release_above(cur);
                     bc[BC_D] = m;
                 }
                 __syncthreads();
@@ -1288,7 +1324,7 @@ struct DevHeap {
                 l2g_batch<K, B, NT>(node(1), X0);
                 publish();
                 if (threadIdx.x == 0) {
-                    release_st(cur >> 1, AVAIL);
+                    release_above(cur);
                     if (tgt != cur) release_st(cur, AVAIL);
                     lock_cas(tgt, MARKED, TARGET);
                 }
@@ -1297,7 +1333,7 @@ struct DevHeap {
             l2g_batch<K, B, NT>(node(tgt), X0);
             publish();
             if (threadIdx.x == 0) {
-                release_st(cur >> 1, AVAIL);
+                release_above(cur);
                 release_st(cur, AVAIL);
             }
         }
@@ -1310,9 +1346,9 @@ struct DevHeap {
         if (!op_begin()) return 0;
         if (threadIdx.x == 0) {
             bc[BC_F] = 0;
-            if (lock_cas(0, AVAIL, INUSE)) {
+            if (lock0()) {
                 bc[BC_F] = a_load(ctr(CTR_PARTIAL)) + a_load(ctr(CTR_BATCH_COUNT)) * B;
-                release_st(0, AVAIL);
+                unlock0();
             }
         }
         __syncthreads();

@@ -41,14 +41,6 @@ namespace bgpq {
 constexpr int kSC1 = 16;  // CPol::SC1 on gfx940+: device-coherent
 
 __device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
-// vmcnt(N): every vector-memory access of this lane but the N newest is done.
-// Stores retire in issue order, so with only stores in flight this waits for
-// the older ones while the N newest may still be on their way.
-template <int N>
-__device__ __forceinline__ void wait_vmem_but() {
-    static_assert(N >= 0 && N < 16, "vmcnt field");
-    __builtin_amdgcn_s_waitcnt(0x0F70 | N);
-}
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
@@ -319,10 +311,27 @@ __device__ __forceinline__ void load_run(K (&v)[N], const K* src) {
 }
 
 // ---------------------------------------------------------------- bitonic sort
+// Position of thread t's run of P keys in the sort's LDS exchange.  LDS has
+// 32 banks of 4 bytes.  A run of 32 bytes (P = 8 u32 or 4 u64 keys) moves as
+// two 16-byte accesses, which the LDS serves 8 lanes at a time; runs 8 words
+// apart would put lanes t and t + 4 on the same banks (a 2-way conflict), so
+// 16 bytes of padding follow every 4 runs and the 8 lanes cover all 32
+// banks.  Shorter runs are conflict-free unpadded.  The exchange then needs
+// B*sizeof(K) + NT*4 bytes.
+template <typename K, int P>
+__device__ __forceinline__ int xch_pos(int t) {
+    if constexpr (P * sizeof(K) == 32) return t * P + (t >> 2) * (16 / (int)sizeof(K));
+    else return t * P;
+}
+template <typename K, int B, int NT>
+constexpr int sort_xch_bytes() {
+    return (B / NT) * (int)sizeof(K) == 32 ? B * (int)sizeof(K) + NT * 4 : B * (int)sizeof(K);
+}
+
 // Ascending sort of the B = P*NT keys held blocked in registers.  `xch` is an
-// LDS scratch of B keys used for the cross-wave strides (may alias nothing
-// that is live).  Semantics match ibitonicSort (heap/heaputil.cuh:23-47):
-// the result is the sorted multiset.
+// LDS scratch of sort_xch_bytes() used for the cross-wave strides (may alias
+// nothing that is live).  Semantics match ibitonicSort (heap/heaputil.cuh:
+// 23-47): the result is the sorted multiset.
 template <typename K, int P, int NT>
 __device__ __forceinline__ void wg_sort_regs(K (&v)[P], K* xch) {
     constexpr int B = P * NT;
@@ -363,10 +372,10 @@ __device__ __forceinline__ void wg_sort_regs(K (&v)[P], K* xch) {
                         v[i] = kminmax(v[i], w, take_min);
                     }
                 } else {
-                    store_run<K, P>(xch + t * P, v);
+                    store_run<K, P>(xch + xch_pos<K, P>(t), v);
                     __syncthreads();
                     K w[P];
-                    load_run<K, P>(w, xch + (t ^ m) * P);
+                    load_run<K, P>(w, xch + xch_pos<K, P>(t ^ m));
                     __syncthreads();
 #pragma unroll
                     for (int i = 0; i < P; ++i) v[i] = kminmax(v[i], w[i], take_min);
@@ -377,7 +386,8 @@ __device__ __forceinline__ void wg_sort_regs(K (&v)[P], K* xch) {
 }
 
 // Load `size` keys (size <= B) from `src`, pad the rest with `pad`, sort,
-// and leave the sorted batch in LDS `dst` (B keys).  `src` is global memory.
+// and leave the sorted batch in LDS `dst` (B keys; the sort's exchange uses
+// sort_xch_bytes() from dst).  `src` is global memory or LDS.
 template <typename K, int B, int NT>
 __device__ __forceinline__ void wg_sort_batch(K* dst, const K* src, int size, K pad) {
     constexpr int P = B / NT;
@@ -694,20 +704,25 @@ __device__ __forceinline__ void wg_merge_bitonic(K* a, K* b, const DS& small, co
 }
 
 // Group merge.  The 2B outputs are cut into windows of W = G*L keys, one
-// per group of G = 16 consecutive lanes (L = 2B/NT keys per lane):
+// per group of G = 32 consecutive lanes (L = 2B/NT keys per lane):
 //  1. The group finds the merge-path split of its window's first output with
 //     a cooperative (G+1)-ary search: each round, lane j probes position
 //     lo + (j+1)*n/(G+1) and a ballot counts the probes that still lie
-//     before the split (3 rounds for B <= 4096, 4 for 8192), with no barrier.
+//     before the split (3 rounds up to B = 8192), with no barrier.
 //  2. Past the split, the window holds the W smallest of the next W keys of
 //     each input.  c[p] = min(a[ai+p], b[bi+W-1-p]) (p < W) is bitonic and
 //     holds exactly them (Batcher); log2(W) half-cleaning stages sort it.
 // Lane j of a group owns window positions p = s*G + j (s < L): every LDS
 // read and write of the group is G consecutive words, free of bank
-// conflicts; strides >= G of the cleaning stages stay in the lane, smaller
-// ones use DPP / ds_swizzle.  No step depends on a previous output, so the
+// conflicts (the split search's probes aside); strides >= G of the cleaning
+// stages stay in the lane, smaller ones use DPP / ds_swizzle.  No step depends on a previous output, so the
 // dependent chain is the search plus one LDS round trip.
-constexpr int kMergeGroup = 16;
+// G = 32: the LDS serves a wave's 4-byte accesses 32 lanes at a time, and
+// 32 lanes reading 32 consecutive words hit 32 distinct banks wherever the
+// window starts.  (With 16-lane groups two groups of unrelated offsets
+// shared each 32-lane pass and met on the same banks: measured as 31-35% of
+// the merges' LDS-active cycles.)
+constexpr int kMergeGroup = 32;
 
 template <typename K, int B, int NT, int GS = kMergeGroup>
 struct GroupMerge {
@@ -846,12 +861,6 @@ __device__ __forceinline__ void quad_transpose(K (&v)[L]) {
 // To global memory with L % 4 == 0, a quad transpose first gives lane
 // 4q + r the four consecutive keys off + (4m + r)*G + 4q .. +3 of each block
 // m, which go out as 16-byte stores (u32; two for u64) instead of 4-byte ones.
-// Global store instructions put_strided issues per lane (kept in step with
-// its branches below).
-template <typename K, int L, int G>
-constexpr int strided_global_stores() {
-    return (L % 4 == 0 && G % 4 == 0) ? (sizeof(K) == 4 ? L / 4 : L / 2) : L;
-}
 template <typename K, int L, int G, typename D>
 __device__ __forceinline__ void put_strided(const D& dst, int off, int j, const K (&v)[L]) {
     if constexpr (std::is_same<D, ToNone>::value) {
@@ -933,8 +942,6 @@ struct HalfMerge {
         else put_run<K, N>(dst, t * N, v);
     }
     __device__ __forceinline__ K last() const { return v[N - 1]; }
-    // vector-memory store instructions one lane issues in put(ToGlobal)
-    static constexpr int kGlobalStores = kGroup ? strided_global_stores<K, N, G>() : -1;
 };
 
 // The merge used by the heap, by measurement on MI355X (one workgroup,

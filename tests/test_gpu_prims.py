@@ -74,10 +74,3 @@ def test_merge_batches(gpu, B, key, algo):
         u = np.sort(np.concatenate([a[i], b[i]]))
         assert np.array_equal(s[i], u[:B]), f"small half of batch {i}"
         assert np.array_equal(l_[i], u[B:]), f"large half of batch {i}"
-
-
-def test_primbench_runs(gpu):
-    """The counter-collection entry point (python -m gpuheap_amd.primbench)
-    runs the merges and the sort and checks the first merge's small half."""
-    from gpuheap_amd import primbench
-    primbench.main(["primbench", "2048", "8"])
