@@ -12,8 +12,9 @@ knapsack queries, each with its own device heap.
   heap, as the reference's applications run.
 * ``run_served`` feeds the persistent serve kernel with Poisson arrivals at
   a given rate and records each query's latency (arrival to completion).
-* ``autotune`` picks the batch size and team size of every size class by
-  timing concurrent copies of a sample query through the batch path.
+* ``autotune`` picks the batch size and team size of every size class from
+  the latency of the class's costliest query alone on the serve kernel,
+  among the configurations that use their workgroups efficiently.
 """
 import json
 import time
@@ -221,37 +222,65 @@ def run_served(cat, stream, table, rate, budget, workgroups=0, seed=2, timeout=6
 
 
 # ---------------------------------------------------------------- autotuner
-def autotune(cat, budget, batch_sizes=(256, 1024), teams=(1, 4, 16), workgroups=0, copies_wg=256, log=None):
-    """For every size class, time `copies` concurrent copies of one sample
-    query per (batch size, team) through the batch path and keep the
-    configuration with the highest queries/s.  copies = copies_wg // team,
-    so each trial offers the kernel the same number of workgroup slots."""
+def _solve_alone(srv, q, timeout=120.0):
+    """Submit one query to an idle server and wait for it: its latency (s)
+    and its result."""
+    while not srv.submit(q):
+        srv.poll(64)
+        time.sleep(0.0005)
+    t0 = S.now()
+    while True:
+        got = srv.poll(8)
+        if got:
+            return got[0]["t_done"] - got[0]["t_submit"], got[0]
+        if S.now() - t0 > timeout:
+            raise RuntimeError("autotune: a query did not complete")
+        time.sleep(0.0002)
+
+
+def autotune(cat, budget, batch_sizes=(256, 1024), teams=(1, 2, 4, 8, 16, 32, 64), workgroups=0, work_slack=1.5,
+             reps=2, log=None):
+    """For every size class, the batch size and team that solve the class's
+    costliest sample query (each sample timed once at B = 1024, team 8) with
+    the lowest latency, among the configurations whose work (team x latency,
+    in workgroup-seconds) is within `work_slack` of the least.  Large queries
+    get the workgroups that shorten them while the workgroups stay busy with
+    useful work; that keeps both the stream's tail latency and its makespan
+    short.  Each configuration is timed `reps` times, alone on the serve
+    kernel (the median counts)."""
     classes = {}
     for kind, key in cat.keys():
-        classes.setdefault(size_class(kind, key), (kind, key))
+        classes.setdefault(size_class(kind, key), []).append((kind, key))
     table = {}
-    for cls, (kind, key) in sorted(classes.items()):
-        best = None
-        for B in batch_sizes:
-            for team in teams:
-                copies = max(1, copies_wg // team)
-                q0 = cat.query(kind, key, B, team, 0)
-                per = S.query_bytes(q0)
-                copies = int(min(copies, max(1, (budget // 2) // per)))
-                qs = [cat.query(kind, key, B, team, i) for i in range(copies)]
-                try:
-                    _, ms = S.batch_run(qs, budget, workgroups=workgroups, repeats=1)
-                except Exception as e:  # a configuration that fails is skipped
+    with S.Server(budget, workgroups=workgroups) as srv:
+        for cls, members in sorted(classes.items()):
+            kind = members[0][0]
+            # the costliest member is the sample
+            sample, worst = members[0], -1.0
+            if len(members) > 1:
+                for k, key in members:
+                    t, _ = _solve_alone(srv, cat.query(k, key, 1024, 8, 0))
+                    if t > worst:
+                        sample, worst = (k, key), t
+            trials = []
+            for B in batch_sizes:
+                for team in teams:
+                    ts = []
+                    for r in range(reps):
+                        t, res = _solve_alone(srv, cat.query(kind, sample[1], B, team, r))
+                        if res["status"] != 0:
+                            ts = None
+                            break
+                        ts.append(t)
+                    if ts is None:
+                        continue
+                    t = float(np.median(ts))
+                    trials.append((t, team * t, B, team))
                     if log:
-                        log(f"[autotune] {cls} B={B} team={team}: {e}")
-                    continue
-                qps = copies / (ms / 1e3)
-                if log:
-                    log(f"[autotune] {cls} B={B} team={team} copies={copies}: {ms:.2f} ms, {qps:.1f} q/s")
-                if best is None or qps > best[0]:
-                    best = (qps, B, team)
-        if best:
-            table[f"{cls[0]}:{cls[1]}"] = {"B": best[1], "team": best[2], "qps": round(best[0], 2)}
+                        log(f"[autotune] {cls} {sample[1]} B={B} team={team}: {t * 1e3:.2f} ms")
+            least = min(w for _, w, _, _ in trials)
+            t, w, B, team = min(x for x in trials if x[1] <= work_slack * least)
+            table[f"{cls[0]}:{cls[1]}"] = {"B": B, "team": team, "ms": round(t * 1e3, 3), "wg_ms": round(w * 1e3, 3)}
     return table
 
 

@@ -284,6 +284,10 @@ __device__ __forceinline__ void store_run(K* dst, const K (&v)[N]) {
             }
             reinterpret_cast<uint4*>(dst)[c] = q;
         }
+    } else if constexpr (N * sizeof(K) == 8 && sizeof(K) == 4) {
+        // one 8-byte access: two 4-byte ones two words apart would put lanes
+        // t and t + 16 of an LDS pass on the same bank
+        reinterpret_cast<uint2*>(dst)[0] = make_uint2((uint32_t)v[0], (uint32_t)v[1]);
     } else {
 #pragma unroll
         for (int i = 0; i < N; ++i) dst[i] = v[i];
@@ -304,9 +308,48 @@ __device__ __forceinline__ void load_run(K (&v)[N], const K* src) {
                 v[c * 2 + 1] = (K)(((uint64_t)q.w << 32) | q.z);
             }
         }
+    } else if constexpr (N * sizeof(K) == 8 && sizeof(K) == 4) {
+        const uint2 q = reinterpret_cast<const uint2*>(src)[0];
+        v[0] = (K)q.x;
+        v[1] = (K)q.y;
     } else {
 #pragma unroll
         for (int i = 0; i < N; ++i) v[i] = src[i];
+    }
+}
+
+// The same moves for runs in LDS.  A 32-byte run goes as two 16-byte
+// accesses, which the LDS serves 8 lanes at a time; runs 8 words apart put
+// lanes t and t + 4 on the same banks, so lanes of odd quads move their
+// second half first and every access instruction covers all 32 banks.
+template <typename K, int N>
+__device__ __forceinline__ void store_run_lds(K* dst, const K (&v)[N]) {
+    if constexpr (N * sizeof(K) == 32) {
+        K w[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) w[i] = v[i];
+        uint4 q[2];
+        store_run<K, N>(reinterpret_cast<K*>(q), w);
+        const int sw = ((int)threadIdx.x >> 2) & 1;
+        uint4* d = reinterpret_cast<uint4*>(dst);
+        d[sw] = sw ? q[1] : q[0];
+        d[sw ^ 1] = sw ? q[0] : q[1];
+    } else {
+        store_run<K, N>(dst, v);
+    }
+}
+template <typename K, int N>
+__device__ __forceinline__ void load_run_lds(K (&v)[N], const K* src) {
+    if constexpr (N * sizeof(K) == 32) {
+        const int sw = ((int)threadIdx.x >> 2) & 1;
+        const uint4* s = reinterpret_cast<const uint4*>(src);
+        const uint4 x = s[sw], y = s[sw ^ 1];
+        uint4 q[2];
+        q[0] = sw ? y : x;
+        q[1] = sw ? x : y;
+        load_run<K, N>(v, reinterpret_cast<const K*>(q));
+    } else {
+        load_run<K, N>(v, src);
     }
 }
 
@@ -403,7 +446,7 @@ __device__ __forceinline__ void wg_sort_batch(K* dst, const K* src, int size, K 
         }
     }
     wg_sort_regs<K, P, NT>(v, dst);
-    store_run<K, P>(dst + t * P, v);
+    store_run_lds<K, P>(dst + t * P, v);
     __syncthreads();
 }
 
@@ -653,8 +696,8 @@ __device__ __forceinline__ void wg_merge_bitonic(K* a, K* b, const DS& small, co
     constexpr int P = B / NT;
     const int t = threadIdx.x;
     K c[P], d[P], av[P], bv[P];
-    load_run<K, P>(av, a + t * P);
-    load_run<K, P>(bv, b + (B - P - t * P));
+    load_run_lds<K, P>(av, a + t * P);
+    load_run_lds<K, P>(bv, b + (B - P - t * P));
 #pragma unroll
     for (int i = 0; i < P; ++i) {
         const K x = av[i], y = bv[P - 1 - i];
@@ -682,12 +725,12 @@ __device__ __forceinline__ void wg_merge_bitonic(K* a, K* b, const DS& small, co
             if (j < 64 * P) {
                 xlane_minmax<K, P, NT>(c, d, m, lower);
             } else {
-                store_run<K, P>(a + t * P, c);
-                store_run<K, P>(b + t * P, d);
+                store_run_lds<K, P>(a + t * P, c);
+                store_run_lds<K, P>(b + t * P, d);
                 __syncthreads();
                 K wc[P], wd[P];
-                load_run<K, P>(wc, a + (t ^ m) * P);
-                load_run<K, P>(wd, b + (t ^ m) * P);
+                load_run_lds<K, P>(wc, a + (t ^ m) * P);
+                load_run_lds<K, P>(wd, b + (t ^ m) * P);
                 __syncthreads();
 #pragma unroll
                 for (int i = 0; i < P; ++i) {
@@ -704,11 +747,12 @@ __device__ __forceinline__ void wg_merge_bitonic(K* a, K* b, const DS& small, co
 }
 
 // Group merge.  The 2B outputs are cut into windows of W = G*L keys, one
-// per group of G = 32 consecutive lanes (L = 2B/NT keys per lane):
+// per group of G consecutive lanes (merge_group: 128 bytes of keys; L =
+// 2B/NT keys per lane):
 //  1. The group finds the merge-path split of its window's first output with
 //     a cooperative (G+1)-ary search: each round, lane j probes position
 //     lo + (j+1)*n/(G+1) and a ballot counts the probes that still lie
-//     before the split (3 rounds up to B = 8192), with no barrier.
+//     before the split (2-4 rounds), with no barrier.
 //  2. Past the split, the window holds the W smallest of the next W keys of
 //     each input.  c[p] = min(a[ai+p], b[bi+W-1-p]) (p < W) is bitonic and
 //     holds exactly them (Batcher); log2(W) half-cleaning stages sort it.
@@ -717,14 +761,19 @@ __device__ __forceinline__ void wg_merge_bitonic(K* a, K* b, const DS& small, co
 // conflicts (the split search's probes aside); strides >= G of the cleaning
 // stages stay in the lane, smaller ones use DPP / ds_swizzle.  No step depends on a previous output, so the
 // dependent chain is the search plus one LDS round trip.
-// G = 32: the LDS serves a wave's 4-byte accesses 32 lanes at a time, and
-// 32 lanes reading 32 consecutive words hit 32 distinct banks wherever the
-// window starts.  (With 16-lane groups two groups of unrelated offsets
-// shared each 32-lane pass and met on the same banks: measured as 31-35% of
-// the merges' LDS-active cycles.)
-constexpr int kMergeGroup = 32;
+// The group spans one LDS pass: the LDS serves 128 bytes (32 banks of 4
+// bytes) per cycle, so a wave's 4-byte accesses go 32 lanes at a time and
+// its 8-byte accesses 16 lanes at a time.  A group of that many lanes reads
+// 128 consecutive bytes, which hit 32 distinct banks wherever its window
+// starts: G = 32 for u32 keys, 16 for u64.  (Two 16-lane groups of
+// unrelated offsets in one 32-lane pass of u32 reads met on the same banks:
+// 31-35% of the merges' LDS-active cycles.  For u64 keys 32-lane groups
+// only add a cleaning stage: a served single-workgroup SSSP solve took 45%
+// longer with them.)
+template <typename K>
+constexpr int merge_group() { return 128 / (int)sizeof(K); }
 
-template <typename K, int B, int NT, int GS = kMergeGroup>
+template <typename K, int B, int NT, int GS = merge_group<K>()>
 struct GroupMerge {
     static constexpr int G = GS;
     static constexpr int L = 2 * B / NT;
@@ -748,15 +797,21 @@ struct GroupMerge {
         int lo = D > B ? D - B : 0;
         int hi = D < B ? D : B;
         const int shift = ((int)__lane_id() / G) * G;
-        // split = first q in [lo, hi] with b[D-1-q] < a[q] (or hi)
+        // split = first q in [lo, hi] with b[D-1-q] < a[q] (or hi).  Lane j
+        // probes q_j = lo + (j+1)*s - 1 with an odd stride s >= n/(G+1)
+        // (probes at or past hi count as not before the split), so the G
+        // probes of a group fall on G distinct banks in both inputs and
+        // each round leaves an interval of at most s; with n <= G + 1 the
+        // probes are lo, lo+1, ... and at most one more round follows.
         while (lo < hi) {
             const int n = hi - lo;
-            const int q = lo + ((j + 1) * n) / (G + 1);
-            const bool before = !(b[D - 1 - q] < a[q]);
+            const int st = ((n + G) / (G + 1)) | 1;
+            const int q = lo + (j + 1) * st - 1;
+            const bool before = q < hi && !(b[D - 1 - kmin(q, hi - 1)] < a[kmin(q, hi - 1)]);
             const uint64_t m = __ballot(before);
             const int c = __popcll((m >> shift) & ((1ull << G) - 1));
-            const int nlo = c > 0 ? lo + (c * n) / (G + 1) + 1 : lo;
-            const int nhi = c < G ? lo + ((c + 1) * n) / (G + 1) : hi;
+            const int nlo = lo + c * st;
+            const int nhi = (c < G && lo + (c + 1) * st - 1 < hi) ? lo + (c + 1) * st - 1 : hi;
             lo = nlo;
             hi = nhi;
         }
@@ -897,7 +952,7 @@ __device__ __forceinline__ void put_strided(const D& dst, int off, int j, const 
 // Merge a and b (LDS) with the group merge and write the B smallest keys to
 // `small`, the B largest to `large`.  Destinations may alias a or b.  Ends
 // with a barrier.
-template <typename K, int B, int NT, typename DS, typename DL, int GS = kMergeGroup>
+template <typename K, int B, int NT, typename DS, typename DL, int GS = merge_group<K>()>
 __device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& small, const DL& large) {
     using GM = GroupMerge<K, B, NT, GS>;
     K out[GM::L];
@@ -922,7 +977,7 @@ template <typename K, int B, int NT, bool HI, bool GRP = (B >= 1024)>
 struct HalfMerge {
     static constexpr int N = B / NT;
     static constexpr bool kGroup = GRP;
-    static constexpr int G = kMergeGroup;
+    static constexpr int G = merge_group<K>();
     static_assert(!kGroup || NT % G == 0, "half merge shape");
     K v[N];
 
