@@ -61,6 +61,8 @@ SIGNATURES = {
                            ctypes.POINTER(ctypes.c_uint32), c_void_p, c_void_p]),
     "bgpq_knapsack": (c_int, [c_void_p, ctypes.c_int32, c_void_p, c_void_p, c_u64, c_i64, c_int,
                               ctypes.POINTER(c_u64), c_void_p, c_void_p]),
+    "bgpq_knapsack_noalloc": (c_int, [c_void_p, ctypes.c_int32, c_void_p, c_void_p, c_u64, c_i64, c_int,
+                                      ctypes.POINTER(c_u64), c_void_p, c_void_p]),
     "bgpq_label_digest": (c_int, [c_void_p, c_i64, c_void_p, c_void_p]),
     "bgpq_gen_rmat": (c_int, [c_int, c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, c_u64,
                               ctypes.c_uint32, ctypes.c_uint32, c_void_p, c_void_p, c_void_p]),

@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -134,8 +136,9 @@ using namespace bgpq;
 namespace {
 
 // Checks shared by the drivers: a u64 heap with a partial buffer, a batch
-// size with a driver instantiation, empty and without error flags.
-int check_heap(bgpq_heap* h, const AppLaunch** row) {
+// size with a driver instantiation, empty and without error flags.  With
+// alloc_ctl the handle's control words are allocated on first use.
+int check_heap(bgpq_heap* h, const AppLaunch** row, bool alloc_ctl = true) {
     if (!h) return bgpq_fail(BGPQ_EINVAL, "null heap");
     if (h->key_type != BGPQ_KEY_U64) return bgpq_fail(BGPQ_EINVAL, "drivers need a u64 heap");
     *row = app_row(h->variant, h->batch_size);
@@ -147,7 +150,7 @@ int check_heap(bgpq_heap* h, const AppLaunch** row) {
     if (rc) return rc;
     if (c[CTR_BATCH_COUNT] || c[CTR_PARTIAL]) return bgpq_fail(BGPQ_EINVAL, "heap is not empty");
     if (c[CTR_ERROR] || c[CTR_ABORT]) return bgpq_fail(BGPQ_EINVAL, "heap carries error flags 0x%x", c[CTR_ERROR]);
-    if (!h->app_ctl) BGPQ_HIPCHK(hipMalloc(&h->app_ctl, sizeof(long long) * APP_WORDS * kCtlStride));
+    if (alloc_ctl && !h->app_ctl) BGPQ_HIPCHK(hipMalloc(&h->app_ctl, sizeof(long long) * APP_WORDS * kCtlStride));
     return BGPQ_OK;
 }
 
@@ -156,29 +159,40 @@ int check_heap(bgpq_heap* h, const AppLaunch** row) {
 // contention (R-MAT scale 22 SSSP: 157 ms at 64, 266 ms at 256).
 int resolve_grid(int grid) { return grid > 0 ? grid : 64; }
 
-// Timed launch, then the control words and heap counters are read back.
-// A threshold search (A*) ends with its pruned keys still in the heap:
-// there must be exactly `outstanding` of them, and the heap is cleared.
+// Solve launch (timed with HIP events, or with the host clock around the
+// stream's completion when events=false), then the control words `ctl` and
+// the heap counters are read back.  A threshold search (A*) ends with its
+// pruned keys still in the heap: there must be exactly `outstanding` of
+// them, and the heap is cleared.
 template <class F>
-int run_solve(bgpq_heap* h, hipStream_t s, bgpq_app_stats* stats, F launch, bool threshold = false) {
-    hipEvent_t e0, e1;
-    BGPQ_HIPCHK(hipEventCreate(&e0));
-    BGPQ_HIPCHK(hipEventCreate(&e1));
-    BGPQ_HIPCHK(hipEventRecord(e0, s));
-    hipError_t e = launch();
-    if (e != hipSuccess) {
+int run_solve(bgpq_heap* h, long long* ctl, hipStream_t s, bgpq_app_stats* stats, F launch, bool threshold = false,
+              bool events = true) {
+    float ms = 0.f;
+    if (events) {
+        hipEvent_t e0, e1;
+        BGPQ_HIPCHK(hipEventCreate(&e0));
+        BGPQ_HIPCHK(hipEventCreate(&e1));
+        BGPQ_HIPCHK(hipEventRecord(e0, s));
+        hipError_t e = launch();
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return bgpq_fail(BGPQ_EHIP, "driver launch: %s", hipGetErrorString(e));
+        }
+        BGPQ_HIPCHK(hipEventRecord(e1, s));
+        BGPQ_HIPCHK(hipEventSynchronize(e1));
+        BGPQ_HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        return bgpq_fail(BGPQ_EHIP, "driver launch: %s", hipGetErrorString(e));
+    } else {
+        const auto t0 = std::chrono::steady_clock::now();
+        hipError_t e = launch();
+        if (e != hipSuccess) return bgpq_fail(BGPQ_EHIP, "driver launch: %s", hipGetErrorString(e));
+        BGPQ_HIPCHK(hipStreamSynchronize(s));
+        ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    BGPQ_HIPCHK(hipEventRecord(e1, s));
-    BGPQ_HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    BGPQ_HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    std::vector<long long> raw((size_t)APP_WORDS * kCtlStride);
-    BGPQ_HIPCHK(hipMemcpy(raw.data(), h->app_ctl, sizeof(long long) * raw.size(), hipMemcpyDeviceToHost));
+    long long raw[APP_WORDS * kCtlStride];
+    BGPQ_HIPCHK(hipMemcpy(raw, ctl, sizeof raw, hipMemcpyDeviceToHost));
     auto word = [&](int i) { return raw[(size_t)i * kCtlStride]; };
     int c[CTR_WORDS];
     int rc = bgpq_handle_counters(h, c);
@@ -227,7 +241,8 @@ extern "C" int bgpq_sssp(bgpq_heap* h, const int32_t* d_rowptr, const int32_t* d
     const SsspApp app{d_rowptr, d_col, d_w, d_dist, h->app_ctl};
     const HeapArgs a = bgpq_handle_args(h, 0);
     const int g = resolve_grid(grid);
-    return run_solve(h, s, stats, [&] { return row->sssp(a, app, h->app_ctl, (uint64_t)(uint32_t)source, g, s); });
+    return run_solve(h, h->app_ctl, s, stats,
+                     [&] { return row->sssp(a, app, h->app_ctl, (uint64_t)(uint32_t)source, g, s); });
 }
 
 extern "C" int bgpq_astar(bgpq_heap* h, const uint8_t* d_free, int32_t width, int32_t height, uint32_t* d_g, int grid,
@@ -251,7 +266,8 @@ extern "C" int bgpq_astar(bgpq_heap* h, const uint8_t* d_free, int32_t width, in
     const uint32_t h0 = 10u * std::max(dx, dy) + 4u * std::min(dx, dy);
     const HeapArgs a = bgpq_handle_args(h, 0);
     const int g = resolve_grid(grid);
-    rc = run_solve(h, s, stats, [&] { return row->astar(a, app, h->app_ctl, (uint64_t)h0 << 32, g, s); }, true);
+    rc = run_solve(h, h->app_ctl, s, stats, [&] { return row->astar(a, app, h->app_ctl, (uint64_t)h0 << 32, g, s); },
+                   true);
     if (rc) return rc;
     BGPQ_HIPCHK(hipMemcpy(cost, d_g + (cells - 1), sizeof(uint32_t), hipMemcpyDeviceToHost));
     return BGPQ_OK;
@@ -338,7 +354,7 @@ extern "C" int bgpq_knapsack(bgpq_heap* h, int32_t n, const uint32_t* weight, co
     const uint64_t first = ((((kUbTop - ub0) << 8) | 255ull) << 32) | 0ull;
     const HeapArgs a = bgpq_handle_args(h, 0);
     const int g = resolve_grid(grid);
-    rc = run_solve(h, s, stats, [&] { return row->knapsack(a, app, h->app_ctl, first, g, s); });
+    rc = run_solve(h, h->app_ctl, s, stats, [&] { return row->knapsack(a, app, h->app_ctl, first, g, s); });
     if (rc) return release(rc);
     long long b = 0;
     if (hipMemcpy(&b, h->app_ctl + (size_t)APP_BEST * kCtlStride, sizeof b, hipMemcpyDeviceToHost) != hipSuccess)
@@ -357,5 +373,152 @@ extern "C" int bgpq_label_digest(const uint32_t* d_labels, int64_t n, uint64_t* 
     hipLaunchKernelGGL(k_digest, dim3((unsigned)blocks), dim3(256), 0, s, d_labels, (long long)n,
                        (unsigned long long*)d_out2);
     BGPQ_HIPCHK(hipGetLastError());
+    return BGPQ_OK;
+}
+
+// ------------------------------------------------------- no allocation
+// bgpq_knapsack_noalloc keeps all that bgpq_knapsack allocates (the
+// control words, the item arrays, the prefix sums, the dominance cells and
+// the node pool) in this array instead.  Each device the library runs on
+// holds its own copy until the process ends.  In host code `scratch` only
+// names the symbol: kernels get the address hipGetSymbolAddress returns.
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_at[kMaxDevices];  // per device, found on the first call there
+
+// The current device's copy of `scratch`, looked up once per device.
+int scratch_for_current_device(unsigned char** out) {
+    int dev = 0;
+    BGPQ_HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return bgpq_fail(BGPQ_EINVAL, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_at[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return bgpq_fail(BGPQ_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_at[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_at[dev];
+    return BGPQ_OK;
+}
+
+size_t up16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+}  // namespace
+
+extern "C" int bgpq_knapsack_noalloc(bgpq_heap* h, int32_t n, const uint32_t* weight, const uint32_t* value,
+                                     uint64_t capacity, int64_t pool_nodes, int grid, uint64_t* best,
+                                     bgpq_app_stats* stats, void* stream) {
+    const AppLaunch* row = nullptr;
+    int rc = check_heap(h, &row, false);
+    if (rc) return rc;
+    if (n < 1 || n > 255 || !weight || !value || !best) return bgpq_fail(BGPQ_EINVAL, "knapsack: bad arguments");
+    uint64_t vsum = 0;
+    for (int i = 0; i < n; ++i) {
+        if (weight[i] == 0) return bgpq_fail(BGPQ_EINVAL, "knapsack: zero weight");
+        vsum += value[i];
+    }
+    if (vsum >= kUbTop) return bgpq_fail(BGPQ_EINVAL, "knapsack: total value %llu exceeds 2^24 - 2", vsum);
+    if (capacity >= 0xFFFFFFFFull) return bgpq_fail(BGPQ_EINVAL, "knapsack: capacity %llu too large", capacity);
+    // the layout of bgpq_knapsack's allocation, after the control words
+    const uint64_t cells = (uint64_t)(n + 1) * (capacity + 1);
+    const size_t o_w = (size_t)APP_WORDS * kCtlStride * sizeof(long long), o_v = o_w + 4 * (size_t)n;
+    const size_t o_pw = up16(o_v + 4 * (size_t)n), o_pv = o_pw + 8 * (size_t)(n + 1);
+    const size_t o_cell = up16(o_pv + 8 * (size_t)(n + 1));
+    if (cells > kScratchBytes / 4 || up16(o_cell + 4 * cells) + sizeof(KnapNode) > kScratchBytes)
+        return bgpq_fail(BGPQ_EINVAL,
+                         "knapsack_noalloc: %llu dominance cells ((n + 1) * (capacity + 1)) and a node pool do not fit "
+                         "the %zu-byte scratch array; use bgpq_knapsack",
+                         (unsigned long long)cells, kScratchBytes);
+    const size_t o_pool = up16(o_cell + 4 * cells);
+    const int64_t pool_max = (int64_t)((kScratchBytes - o_pool) / sizeof(KnapNode));
+    if (pool_nodes <= 0) pool_nodes = pool_max;
+    if (pool_nodes > pool_max)
+        return bgpq_fail(BGPQ_EINVAL, "knapsack_noalloc: %lld pool nodes exceed the %lld that fit the scratch array",
+                         (long long)pool_nodes, (long long)pool_max);
+    unsigned char* base = nullptr;
+    rc = scratch_for_current_device(&base);
+    if (rc) return rc;
+    // items by value/weight, highest first, ties by index (as bgpq_knapsack;
+    // an insertion sort on the stack)
+    int ord[255];
+    uint32_t w[255], v[255];
+    uint64_t pw[256], pv[256];
+    for (int i = 0; i < n; ++i) {
+        int j = i;
+        while (j > 0 && (uint64_t)value[i] * weight[ord[j - 1]] > (uint64_t)value[ord[j - 1]] * weight[i]) {
+            ord[j] = ord[j - 1];
+            --j;
+        }
+        ord[j] = i;
+    }
+    pw[0] = pv[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        w[i] = weight[ord[i]];
+        v[i] = value[ord[i]];
+        pw[i + 1] = pw[i] + w[i];
+        pv[i + 1] = pv[i] + v[i];
+    }
+    hipStream_t s = (hipStream_t)stream;
+    long long* ctl = reinterpret_cast<long long*>(base);
+    const KnapNode root{0, 0, 0, 0};
+    const uint32_t one = 1;
+    // every call starts from zeroed dominance cells; the control words are
+    // set by k_app_setup, the pool's root below, the rest is written before
+    // it is read
+    BGPQ_HIPCHK(fill_u32(base + o_cell, (long long)cells, 0u, s));
+    if (hipMemcpyAsync(base + o_w, w, 4 * (size_t)n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(base + o_v, v, 4 * (size_t)n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(base + o_pw, pw, 8 * (size_t)(n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(base + o_pv, pv, 8 * (size_t)(n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(base + o_cell, &one, 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(base + o_pool, &root, sizeof root, hipMemcpyHostToDevice, s) != hipSuccess) {
+        (void)hipStreamSynchronize(s);  // no copy may still read the arrays on this stack
+        return bgpq_fail(BGPQ_EHIP, "knapsack_noalloc: upload failed");
+    }
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_app_setup, dim3(1), dim3(64), 0, s, ctl, 1ll, nullptr, 0u, h->ctr, 0);
+    if (hipGetLastError() != hipSuccess) {
+        (void)hipStreamSynchronize(s);
+        return bgpq_fail(BGPQ_EHIP, "knapsack_noalloc: setup launch");
+    }
+    const KnapsackApp app{n,
+                          capacity,
+                          (const uint32_t*)(base + o_w),
+                          (const uint32_t*)(base + o_v),
+                          (const uint64_t*)(base + o_pw),
+                          (const uint64_t*)(base + o_pv),
+                          (uint32_t*)(base + o_cell),
+                          (KnapNode*)(base + o_pool),
+                          (long long)pool_nodes,
+                          ctl};
+    uint64_t ub0 = 0, room = capacity;
+    for (int i = 0; i < n; ++i) {
+        if (w[i] <= room) {
+            room -= w[i];
+            ub0 += v[i];
+        } else {
+            ub0 += room * v[i] / w[i];
+            break;
+        }
+    }
+    const uint64_t first = ((((kUbTop - ub0) << 8) | 255ull) << 32) | 0ull;
+    const HeapArgs a = bgpq_handle_args(h, 0);
+    const int g = resolve_grid(grid);
+    // the host arrays above stay alive: run_solve waits for the stream
+    rc = run_solve(h, ctl, s, stats, [&] { return row->knapsack(a, app, ctl, first, g, s); }, false, false);
+    if (rc) {
+        (void)hipStreamSynchronize(s);
+        return rc;
+    }
+    long long b = 0;
+    BGPQ_HIPCHK(hipMemcpy(&b, ctl + (size_t)APP_BEST * kCtlStride, sizeof b, hipMemcpyDeviceToHost));
+    *best = (uint64_t)b;
     return BGPQ_OK;
 }

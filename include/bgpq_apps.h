@@ -69,6 +69,20 @@ int bgpq_astar(bgpq_heap* h, const uint8_t* d_free, int32_t width, int32_t heigh
 int bgpq_knapsack(bgpq_heap* h, int32_t n, const uint32_t* weight, const uint32_t* value, uint64_t capacity,
                   int64_t pool_nodes, int grid, uint64_t* best, bgpq_app_stats* stats, void* stream);
 
+/* bgpq_knapsack without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid it).  No counterpart in the reference.  Same arguments, checks and
+ * results as bgpq_knapsack, but the control words, item arrays, prefix sums,
+ * dominance cells and node pool live in one 16 MiB array the library keeps
+ * per device for the life of the process (the heap is still the caller's):
+ * an instance whose (n + 1) * (capacity + 1) dominance cells and node pool
+ * do not fit is refused with BGPQ_EINVAL; pool_nodes = 0 takes what fits.
+ * Calls that overlap on one device share that array and race; calls one
+ * after another start from a cleared state.  No HIP events are used: stats->ms
+ * is host time around the solve. */
+int bgpq_knapsack_noalloc(bgpq_heap* h, int32_t n, const uint32_t* weight, const uint32_t* value, uint64_t capacity,
+                          int64_t pool_nodes, int grid, uint64_t* best, bgpq_app_stats* stats, void* stream);
+
 /* Summary of SSSP labels computed on the device, the same one the serving
  * path reports (bgpq_result.digest and .value): d_out2[0] = sum over v of
  * (dist[v] + 1) * (2v + 1) mod 2^64, d_out2[1] = vertices reached
