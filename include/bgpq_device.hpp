@@ -311,6 +311,16 @@ struct DevHeap {
         wait_stores();
         a_add(ctr(CTR_TICKET), 1);
     }
+    // bu, lane 0 under status[0]: batch slot nb + 1 becomes the last one and
+    // is locked for the new batch (its index), or -1 when the heap is full,
+    // -2 on abort.
+    __device__ __forceinline__ int claim_slot(int nb) const {
+        const int idx = nb + 1;
+        if (idx > h.batch_num) return -1;
+        a_store(ctr(CTR_BATCH_COUNT), idx);
+        // the new slot is nearly always free: try the CAS first
+        return (a_cas(st(idx), AVAIL, INUSE) || lock_cas(idx, AVAIL, INUSE)) ? idx : -2;
+    }
     // Release of the node above `cur` in a td insertion walk (status[0]
     // above the root).
     __device__ __forceinline__ void release_above(int cur) const {
@@ -1044,11 +1054,31 @@ struct DevHeap {
         wg_sort_batch<K, B, NT>(X0, items, size, h.init_limits);
         BGPQ_EVENT(21);
 
+        // A batch that goes into the tree (case 2.2, or any insertion
+        // without a partial buffer) needs slot nb + 1.  On overflow nothing
+        // is touched, the partial buffer included: status[0] is released
+        // and the error flagged.
+        auto to_tree_of = [&](int nb_, int pbs_) {
+            return !(PB && (size + pbs_ < B || (nb_ == 0 && size < B)));
+        };
         if (threadIdx.x == 0) {
+            bc[BC_E] = 0;
             if (lock0()) {
                 BGPQ_EVENT(22);
-                bc[BC_A] = a_load(ctr(CTR_BATCH_COUNT));
-                bc[BC_B] = PB ? a_load(ctr(CTR_PARTIAL)) : 0;
+                const int nb0 = a_load(ctr(CTR_BATCH_COUNT));
+                const int pbs0 = PB ? a_load(ctr(CTR_PARTIAL)) : 0;
+                bc[BC_A] = nb0;
+                bc[BC_B] = pbs0;
+                // bu, a batch that goes straight into the tree (no partial
+                // buffer to fold): lane 0 claims its slot and releases
+                // status[0] at once, with no barrier in between
+                if constexpr (BU) {
+                    if (to_tree_of(nb0, pbs0) && !(PB && !(pbs0 == 0 && size == B))) {
+                        bc[BC_C] = claim_slot(nb0);
+                        unlock0();
+                        bc[BC_E] = 1;
+                    }
+                }
             }
             // A workgroup runs one insertion at a time and an insertion leaves
             // no tagged word behind when it ends, so the workgroup index is a
@@ -1058,14 +1088,11 @@ struct DevHeap {
         __syncthreads();
         if (aborted()) return;
         const int nb = bc[BC_A], pbs = bc[BC_B];
+        const bool direct = bc[BC_E] != 0;  // slot claimed, status[0] released
         const int mine = BU ? bc[BC_TAG] : 0;  // this insertion's INSHOLD word
         __syncthreads();
-        // A batch that goes into the tree (case 2.2, or any insertion
-        // without a partial buffer) needs slot nb + 1.  On overflow nothing
-        // is touched, the partial buffer included: status[0] is released
-        // and the error flagged.
-        const bool to_tree = !(PB && (size + pbs < B || (nb == 0 && size < B)));
-        if (to_tree && nb + 1 > h.batch_num) {
+        const bool to_tree = to_tree_of(nb, pbs);
+        if (!direct && to_tree && nb + 1 > h.batch_num) {
             if (threadIdx.x == 0) unlock0();
             flag_error(ERR_OVERFLOW);
             return;
@@ -1116,7 +1143,7 @@ struct DevHeap {
         // 2.2 with an empty partial buffer: the fold below is the identity
         const bool fold = PB && !(pbs == 0 && size == B);
         if constexpr (PB) {
-            if (fold) {
+            if (fold && !direct) {
                 g2l_batch<K, B, NT>(X1, node(0));
                 __syncthreads();
                 if (size + pbs < B) {  // 2.1: fold into the partial buffer via the root
@@ -1146,19 +1173,14 @@ struct DevHeap {
         }
 
         if constexpr (BU) {
-            // ---- claim the next slot and sift up (buitdd:616-689)
-            if (threadIdx.x == 0) {
-                const int idx = nb + 1;
-                if (idx > h.batch_num) {
-                    bc[BC_C] = -1;
-                } else {
-                    a_store(ctr(CTR_BATCH_COUNT), idx);
-                    // the new slot is nearly always free: try the CAS first
-                    bc[BC_C] = (a_cas(st(idx), AVAIL, INUSE) || lock_cas(idx, AVAIL, INUSE)) ? idx : -2;
+            // ---- claim the next slot (unless done above) and sift up (buitdd:616-689)
+            if (!direct) {
+                if (threadIdx.x == 0) {
+                    bc[BC_C] = claim_slot(nb);
+                    unlock0();
                 }
-                unlock0();
+                __syncthreads();
             }
-            __syncthreads();
             if (aborted()) return;
             int cur = bc[BC_C];
             __syncthreads();
