@@ -16,10 +16,40 @@
 
 namespace bgpq {
 
-constexpr int kServeNT = 256;
+// Serve tiers: one persistent kernel per workgroup size, so that every
+// batch size runs at the thread count its heap kernels are built for.
+// Tier 0 (256 threads) takes B = 256 and 512, tier 1 (512 threads) B = 1024:
+// at 256 threads a B = 1024 solve holds twice the keys per lane and spilled
+// registers to scratch.  Each tier has its own slot queue.
+constexpr int kTiers = 2;
+constexpr int kTierNT[kTiers] = {256, 512};
+constexpr int tier_of(int B) { return B == 1024 ? 1 : 0; }
 // Polls (each ~3.4 us of sleep plus a host-memory read) a workgroup waits
 // for the host to post or close before it gives up: about a minute.
 constexpr long long kMaxRingPolls = 1ll << 24;
+
+// What a query's arena block needs before its solve: fills (heap with
+// init_limits, status, counters, control words, labels), copies of small
+// host arrays (knapsack items), and single words set after them.
+constexpr int kMaxFills = 8, kMaxPokes = 4, kMaxCopies = 4;
+struct FillJobs {
+    void* p[kMaxFills];
+    long long n[kMaxFills];
+    unsigned long long v[kMaxFills];
+    int esz[kMaxFills];
+    int count;
+    void* pp[kMaxPokes];
+    unsigned long long pv[kMaxPokes];
+    int pesz[kMaxPokes];
+    int pokes;
+    const void* cs[kMaxCopies];  // pinned host memory
+    void* cd[kMaxCopies];
+    long long cb[kMaxCopies];    // bytes, a multiple of 4
+    int copies;
+};
+// A served query is prepared by its own team, in chunks of this many bytes
+// of fill (and one chunk per copy).
+constexpr long long kSetupChunk = 256 << 10;
 
 // What a workgroup needs to take part in a query, in 8-byte words so that it
 // can be copied from host memory one word per lane.
@@ -46,8 +76,14 @@ struct QueryDesc {
     long long pool_cap;
     uint64_t cap;
     uint64_t first_key;
+    // served queries: the team prepares the block (setup_chunks > 0), with
+    // the chunk counter and the chunks-done count at sync[0] and sync[1]
+    // (device words of the query index, zero between queries)
+    FillJobs setup;
+    int32_t setup_chunks, pad2;
+    unsigned* sync;
 };
-static_assert(sizeof(QueryDesc) % 8 == 0 && sizeof(QueryDesc) / 8 <= kServeNT, "descriptor words");
+static_assert(sizeof(QueryDesc) % 8 == 0 && sizeof(QueryDesc) / 8 <= 256, "descriptor words");
 
 // Written by the last team member to leave, read by the host.
 struct Completion {
@@ -64,11 +100,16 @@ struct ServeRing {
     const int* closed;
     Completion* done;       // per query index
     long long* claim;       // device: member slots taken (only posted ones are)
+    // device: the tail as one workgroup last read it from the ring, with
+    // kClosedBit once the ring was seen closed (then the tail is final)
+    unsigned long long* mirror;
+    int* poller;            // device: 1 while a workgroup is the one reading the ring
     int* fault;             // set when a workgroup left after max_polls idle polls
     int* live;              // workgroups still in the kernel
     long long max_polls;
     unsigned qmask, smask;
 };
+constexpr unsigned long long kClosedBit = 1ull << 62;
 
 // Ring words are read with relaxed system-scope loads, which bypass the
 // caches without invalidating them (an acquire would invalidate the L2 on
@@ -81,30 +122,138 @@ __device__ __forceinline__ int sys_load(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// One solve at batch size B: a call of its own, so that the register
-// allocation of the three batch sizes is not merged into one function.
-template <int B>
-__device__ __attribute__((noinline)) void solve_b(const QueryDesc& d, bool first) {
+// One solve, inlined into its tier's kernel: as calls (noinline) the
+// solves took 216-246 VGPRs at 512 threads, inlined the kernel takes 162,
+// so both tiers' workgroups fit on a CU together.
+template <int B, int NT>
+__device__ __forceinline__ void solve_sssp(const QueryDesc& d, bool first) {
     const HeapView<uint64_t> hv{d.items, d.status, d.ctr, ~0ull, (int)d.batch_num, 0, 0};
-    if (d.type == BGPQ_Q_SSSP) {
-        const SsspApp app{d.rowptr, d.col, d.w, d.labels, d.ctl};
-        drive<VAR_BUITDD, B, kServeNT>(hv, app, d.ctl, d.first_key, first);
-    } else if (d.type == BGPQ_Q_ASTAR) {
-        const AstarApp app{d.free_cells, d.n, d.height, d.labels, d.ctl};
-        drive<VAR_BUITDD, B, kServeNT>(hv, app, d.ctl, d.first_key, first);
-    } else {
-        const KnapsackApp app{d.n, d.cap, d.kw, d.kv, d.pw, d.pv, d.cell, d.pool, d.pool_cap, d.ctl};
-        drive<VAR_BUITDD, B, kServeNT>(hv, app, d.ctl, d.first_key, first);
+    const SsspApp app{d.rowptr, d.col, d.w, d.labels, d.ctl};
+    drive<VAR_BUITDD, B, NT>(hv, app, d.ctl, d.first_key, first);
+}
+template <int B, int NT>
+__device__ __forceinline__ void solve_astar(const QueryDesc& d, bool first) {
+    const HeapView<uint64_t> hv{d.items, d.status, d.ctr, ~0ull, (int)d.batch_num, 0, 0};
+    const AstarApp app{d.free_cells, d.n, d.height, d.labels, d.ctl};
+    drive<VAR_BUITDD, B, NT>(hv, app, d.ctl, d.first_key, first);
+}
+template <int B, int NT>
+__device__ __forceinline__ void solve_knapsack(const QueryDesc& d, bool first) {
+    const HeapView<uint64_t> hv{d.items, d.status, d.ctr, ~0ull, (int)d.batch_num, 0, 0};
+    const KnapsackApp app{d.n, d.cap, d.kw, d.kv, d.pw, d.pv, d.cell, d.pool, d.pool_cap, d.ctl};
+    drive<VAR_BUITDD, B, NT>(hv, app, d.ctl, d.first_key, first);
+}
+template <int B, int NT>
+__device__ __forceinline__ void solve_b(const QueryDesc& d, bool first) {
+    if (d.type == BGPQ_Q_SSSP) solve_sssp<B, NT>(d, first);
+    else if (d.type == BGPQ_Q_ASTAR) solve_astar<B, NT>(d, first);
+    else solve_knapsack<B, NT>(d, first);
+}
+
+// ---- a served query's setup, done by its team (no kernel launch of its
+// own: with the serve kernels resident, a prepare kernel or a copy kernel
+// of the runtime can wait for a free CU until idle workgroups leave).
+// Chunk c of the setup: a kSetupChunk slice of one fill, or one copy.  The
+// stores are device-coherent (sc1, or agent-scope word stores), as every
+// access of the solve is.
+template <int NT>
+__device__ void setup_chunk(const FillJobs& j, unsigned c) {
+    for (int k = 0; k < j.count; ++k) {
+        const long long bytes = j.n[k] * j.esz[k];
+        const unsigned nc = (unsigned)((bytes + kSetupChunk - 1) / kSetupChunk);
+        if (c >= nc) {
+            c -= nc;
+            continue;
+        }
+        const long long begin = (long long)c * kSetupChunk;
+        const int len = (int)(bytes - begin < kSetupChunk ? bytes - begin : kSetupChunk);
+        char* base = (char*)j.p[k] + begin;
+        const uint32_t lo = (uint32_t)j.v[k], hi = j.esz[k] == 8 ? (uint32_t)(j.v[k] >> 32) : lo;
+        const uint4 q = make_uint4(lo, hi, lo, hi);
+        const __amdgpu_buffer_rsrc_t r = buf_rsrc(base);
+        const int n16 = len / 16;
+        for (int i = threadIdx.x; i < n16; i += NT) gstore16(r, i * 16, q);
+        // the rest, in words (begin is a multiple of 16, so word parity
+        // within the slice picks the half of an 8-byte pattern)
+        for (int o = n16 * 16 + 4 * threadIdx.x; o < len; o += 4 * NT)
+            __hip_atomic_store((uint32_t*)(base + o), (o & 4) ? hi : lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    if ((int)c < j.copies) {
+        const uint32_t* src = (const uint32_t*)j.cs[c];
+        uint32_t* dst = (uint32_t*)j.cd[c];
+        for (long long i = threadIdx.x; i < j.cb[c] / 4; i += NT)
+            __hip_atomic_store(dst + i, __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
+// Every member takes chunks until none are left; the member that completes
+// the last one stores the single words and then marks the setup done
+// (sync[1] = chunks + 1), which every member waits for.  A chunk is only
+// ever taken by a running workgroup, so the wait always ends; it is
+// bounded anyway (false: the workgroup gives up).
+template <int NT>
+__device__ bool setup_query(const QueryDesc& d, long long max_polls) {
+    __shared__ unsigned s_c;
+    __shared__ int s_fin;
+    unsigned* next = d.sync;
+    unsigned* done = d.sync + 1;
+    const unsigned total = (unsigned)d.setup_chunks;
+    while (true) {
+        if (threadIdx.x == 0) s_c = __hip_atomic_fetch_add(next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const unsigned c = s_c;
+        __syncthreads();  // every lane has read s_c
+        if (c >= total) break;
+        setup_chunk<NT>(d.setup, c);
+        wait_stores();
+        __syncthreads();
+        if (threadIdx.x == 0)
+            s_fin = __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == total;
+        __syncthreads();
+        if (s_fin) {
+            const int k = threadIdx.x;
+            if (k < d.setup.pokes) {
+                if (d.setup.pesz[k] == 8)
+                    __hip_atomic_store((unsigned long long*)d.setup.pp[k], d.setup.pv[k], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                else
+                    __hip_atomic_store((uint32_t*)d.setup.pp[k], (uint32_t)d.setup.pv[k], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+            wait_stores();
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_store(done, total + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();  // every lane has read s_fin
+    }
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        long long n = 0;
+        while (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != total + 1) {
+            if (++n > max_polls) {
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(8);
+        }
+        s_fin = ok;
+    }
+    __syncthreads();
+    const bool ok = s_fin != 0;
+    __syncthreads();
+    return ok;
+}
+
 // The last member's epilogue: result, digest and statistics to the host.
+template <int NT>
 __device__ void finish(const QueryDesc& d, Completion* c) {
-    __shared__ unsigned long long red[2][kServeNT / 64];
+    __shared__ unsigned long long red[2][NT / 64];
     const int t = threadIdx.x;
     uint64_t dig = 0, reach = 0;
     if (d.type == BGPQ_Q_SSSP) {
-        for (long long v = t; v < d.n_labels; v += kServeNT) {
+        for (long long v = t; v < d.n_labels; v += NT) {
             const uint32_t x = u32_load(d.labels + v);
             dig += ((uint64_t)x + 1) * (2 * (uint64_t)v + 1);
             reach += x != 0xFFFFFFFFu;
@@ -123,7 +272,7 @@ __device__ void finish(const QueryDesc& d, Completion* c) {
         uint64_t value = 0;
         dig = 0;
         if (d.type == BGPQ_Q_SSSP) {
-            for (int i = 0; i < kServeNT / 64; ++i) {
+            for (int i = 0; i < NT / 64; ++i) {
                 dig += red[0][i];
                 value += red[1][i];
             }
@@ -142,6 +291,12 @@ __device__ void finish(const QueryDesc& d, Completion* c) {
             in_heap = (long long)a_load(d.ctr + CTR_BATCH_COUNT * kWordStride) * d.B +
                       a_load(d.ctr + CTR_PARTIAL * kWordStride);
         if (left != in_heap) err |= APPERR_HEAP;
+        // every member is past its setup wait: the query index's words are
+        // cleared for its next query (ordered before the release of `done`)
+        if (d.sync) {
+            __hip_atomic_store(d.sync, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(d.sync + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         auto put = [](auto* p, auto v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); };
         put(&c->value, value);
         put(&c->digest, dig);
@@ -162,7 +317,13 @@ __device__ void finish(const QueryDesc& d, Completion* c) {
 // completion.  A workgroup ends when the ring is closed and every posted
 // slot is taken, or after max_polls unanswered polls (then `fault` is set);
 // either way it leaves the `live` count, which the host reads.
-__global__ __launch_bounds__(kServeNT) void k_serve(ServeRing r) {
+// Idle workgroups poll device words only.  One of them at a time (the
+// holder of `poller`) reads the host's tail and closed flag and raises
+// `mirror` to what it saw: reads of pinned host memory by every idle
+// workgroup of the chip (hundreds, each every few microseconds) slowed the
+// workgroups that were solving about fivefold.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_serve(ServeRing r) {
     __shared__ long long s_slot;
     __shared__ int s_go, s_last;
     __shared__ uint32_t s_entry;
@@ -172,9 +333,11 @@ __global__ __launch_bounds__(kServeNT) void k_serve(ServeRing r) {
         if (t == 0) {
             int go = 0;
             long long s = -1, polls = 0;
+            bool poller = false;
             while (true) {
+                const unsigned long long m = __hip_atomic_load(r.mirror, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const long long c = __hip_atomic_load(r.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (c < sys_load(r.tail)) {
+                if (c < (long long)(m & (kClosedBit - 1))) {
                     long long e = c;
                     if (__hip_atomic_compare_exchange_strong(r.claim, &e, c + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_AGENT)) {
@@ -184,12 +347,22 @@ __global__ __launch_bounds__(kServeNT) void k_serve(ServeRing r) {
                     }
                     continue;  // another workgroup took slot c
                 }
-                // the host posts every slot before it closes the ring, and
-                // this load of the tail issues after the closed flag is seen
-                if (sys_load(r.closed)) {
-                    if (__hip_atomic_load(r.claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < sys_load(r.tail))
-                        continue;
-                    break;
+                if (m & kClosedBit) break;  // the tail is final and every slot is taken
+                if (!poller && __hip_atomic_load(r.poller, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                    int z = 0;
+                    poller = __hip_atomic_compare_exchange_strong(r.poller, &z, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT);
+                }
+                if (poller) {
+                    // the host posts every slot before it closes the ring: a
+                    // tail read after the closed flag was seen is final
+                    const int closed = sys_load(r.closed);
+                    if (closed) __atomic_thread_fence(__ATOMIC_ACQUIRE);
+                    const unsigned long long v = (unsigned long long)sys_load(r.tail) | (closed ? kClosedBit : 0);
+                    if (v > m) {
+                        __hip_atomic_fetch_max(r.mirror, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        continue;  // act on it at once
+                    }
                 }
                 if (++polls > r.max_polls) {
                     __hip_atomic_fetch_or(r.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -197,6 +370,7 @@ __global__ __launch_bounds__(kServeNT) void k_serve(ServeRing r) {
                 }
                 __builtin_amdgcn_s_sleep(127);
             }
+            if (poller) __hip_atomic_store(r.poller, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_go = go;
             s_slot = s;
             if (go)
@@ -214,35 +388,31 @@ __global__ __launch_bounds__(kServeNT) void k_serve(ServeRing r) {
                 __hip_atomic_load(src + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         __syncthreads();
+        if (s_d.setup_chunks > 0 && !setup_query<NT>(s_d, r.max_polls)) {
+            // a member of this query never finished its chunk: leave
+            if (t == 0) {
+                __hip_atomic_fetch_or(r.fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_fetch_add(r.live, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            break;
+        }
         const bool first = member == 0;
-        if (s_d.B == 256) solve_b<256>(s_d, first);
-        else if (s_d.B == 512) solve_b<512>(s_d, first);
-        else solve_b<1024>(s_d, first);
+        if constexpr (NT == kTierNT[1]) {
+            solve_b<1024, NT>(s_d, first);
+        } else {
+            if (s_d.B == 256) solve_b<256, NT>(s_d, first);
+            else solve_b<512, NT>(s_d, first);
+        }
         if (t == 0) {
             wait_stores();
             s_last = __hip_atomic_fetch_add(ctl_word(s_d.ctl, APP_DONE), 1ll, __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_AGENT) == s_d.team - 1;
         }
         __syncthreads();
-        if (s_last) finish(s_d, r.done + (qi & r.qmask));
+        if (s_last) finish<NT>(s_d, r.done + (qi & r.qmask));
         __syncthreads();
     }
 }
-
-// Fills of a query's arena block (heap with init_limits, status, counters,
-// control words, labels) and the single words set after them.
-constexpr int kMaxFills = 8, kMaxPokes = 4;
-struct FillJobs {
-    void* p[kMaxFills];
-    long long n[kMaxFills];
-    unsigned long long v[kMaxFills];
-    int esz[kMaxFills];
-    int count;
-    void* pp[kMaxPokes];
-    unsigned long long pv[kMaxPokes];
-    int pesz[kMaxPokes];
-    int pokes;
-};
 
 __global__ void k_prepare(FillJobs j) {
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -264,8 +434,11 @@ __global__ void k_poke(FillJobs j) {
         else *(uint32_t*)j.pp[k] = (uint32_t)j.pv[k];
     }
 }
-__global__ void k_ring_reset(long long* claim, int* fault, int* live, int workgroups) {
+__global__ void k_ring_reset(long long* claim, unsigned long long* mirror, int* poller, int* fault, int* live,
+                             int workgroups) {
     __hip_atomic_store(claim, 0ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(mirror, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(poller, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(fault, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(live, workgroups, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
@@ -405,10 +578,17 @@ Layout layout_of(const bgpq_query* q) {
     return L;
 }
 
-// Descriptor and fills of one query placed at `base`; knapsack items are
-// uploaded here (on stream s, finished before returning), the fills are
-// left to launch_fills.
-int build_query(const bgpq_query* q, char* base, const Layout& L, hipStream_t s, QueryDesc* d, FillJobs* jobs) {
+// Pinned host bytes a knapsack query's item arrays take while it is
+// prepared (w, v: 4 bytes per item; pw, pv: 8 bytes per item + 8).
+constexpr size_t kStageBytes = 8192;
+static_assert(4 * 255 * 2 + 8 * 256 * 2 <= kStageBytes, "knapsack staging");
+
+// Descriptor and fills of one query placed at `base`.  Batch form: knapsack items are uploaded here (on stream s, finished before
+// returning) and the fills are left to launch_fills.  Served form: the
+// items are written to `stage` (pinned host memory) and everything becomes
+// the descriptor's setup, which the query's team runs (setup_query).
+int build_query(const bgpq_query* q, char* base, const Layout& L, hipStream_t s, QueryDesc* d, FillJobs* jobs,
+                bool served = false, char* stage = nullptr) {
     std::memset(d, 0, sizeof *d);
     d->type = q->type;
     d->B = q->batch_size;
@@ -492,17 +672,36 @@ int build_query(const bgpq_query* q, char* base, const Layout& L, hipStream_t s,
         d->pool = (KnapNode*)(base + L.pool);
         d->pool_cap = L.pool_nodes;
         d->first_key = ((((kUbTop - ub0) << 8) | 255ull) << 32) | 0ull;
-        BGPQ_HIPCHK(hipMemcpyAsync((void*)d->kw, w.data(), 4 * (size_t)n, hipMemcpyHostToDevice, s));
-        BGPQ_HIPCHK(hipMemcpyAsync((void*)d->kv, v.data(), 4 * (size_t)n, hipMemcpyHostToDevice, s));
-        BGPQ_HIPCHK(hipMemcpyAsync((void*)d->pw, pw.data(), 8 * (size_t)(n + 1), hipMemcpyHostToDevice, s));
-        BGPQ_HIPCHK(hipMemcpyAsync((void*)d->pv, pv.data(), 8 * (size_t)(n + 1), hipMemcpyHostToDevice, s));
+        const void* src[4] = {w.data(), v.data(), pw.data(), pv.data()};
+        void* dst[4] = {(void*)d->kw, (void*)d->kv, (void*)d->pw, (void*)d->pv};
+        const size_t len[4] = {4 * (size_t)n, 4 * (size_t)n, 8 * (size_t)(n + 1), 8 * (size_t)(n + 1)};
+        if (served) {
+            if (!stage) return bgpq_fail(BGPQ_EINVAL, "knapsack query: no staging memory");
+            size_t o = 0;
+            for (int k = 0; k < 4; ++k) {
+                std::memcpy(stage + o, src[k], len[k]);
+                j.cs[k] = stage + o;
+                j.cd[k] = dst[k];
+                j.cb[k] = (long long)len[k];
+                o += align_up(len[k], 16);
+            }
+            j.copies = 4;
+        } else {
+            for (int k = 0; k < 4; ++k) BGPQ_HIPCHK(hipMemcpyAsync(dst[k], src[k], len[k], hipMemcpyHostToDevice, s));
+        }
         fill(d->cell, L.cells, 0, 4);
         fill(d->pool, 4, 0, 4);  // the root: depth 0, nothing used, value 0
         poke(d->cell, 1, 4);
         poke(d->ctl + (size_t)APP_POOL * kCtlStride, 1, 8);
-        BGPQ_HIPCHK(hipStreamSynchronize(s));  // the host vectors above go out of scope
+        if (!served) BGPQ_HIPCHK(hipStreamSynchronize(s));  // the host vectors above go out of scope
     }
     *jobs = j;
+    if (served) {
+        d->setup = j;
+        long long chunks = j.copies;
+        for (int k = 0; k < j.count; ++k) chunks += (j.n[k] * j.esz[k] + kSetupChunk - 1) / kSetupChunk;
+        d->setup_chunks = (int32_t)chunks;
+    }
     return BGPQ_OK;
 }
 
@@ -513,16 +712,31 @@ void launch_fills(const FillJobs& j, hipStream_t s, int blocks) {
     hipLaunchKernelGGL(k_poke, dim3(1), dim3(64), 0, s, j);
 }
 
+// Dynamic LDS of a tier's kernel: the workspace of its largest batch size.
+int tier_lds(int t) { return t == 1 ? app_lds_bytes<1024>() : app_lds_bytes<512>(); }
+
 hipError_t allow_serve_lds() {
-    static const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_serve),
-                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                    app_lds_bytes<1024>());
+    static const hipError_t e = [] {
+        hipError_t r = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_serve<kTierNT[0]>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, tier_lds(0));
+        if (r == hipSuccess)
+            r = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_serve<kTierNT[1]>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, tier_lds(1));
+        return r;
+    }();
     return e;
 }
 
-// One serve workgroup per CU by default: a workgroup's solve calls keep up
-// to ~250 VGPRs per lane, and the prepare kernels of new queries must still
-// find room on the CUs while the serve kernel runs.
+// The ring words of one tier, reset, then that tier's serve kernel.
+void launch_tier(int t, int wgs, hipStream_t st, const ServeRing& r) {
+    hipLaunchKernelGGL(k_ring_reset, dim3(1), dim3(1), 0, st, r.claim, r.mirror, r.poller, r.fault, r.live, wgs);
+    if (t == 0) hipLaunchKernelGGL(k_serve<kTierNT[0]>, dim3(wgs), dim3(kTierNT[0]), tier_lds(0), st, r);
+    else hipLaunchKernelGGL(k_serve<kTierNT[1]>, dim3(wgs), dim3(kTierNT[1]), tier_lds(1), st, r);
+}
+
+// One serve workgroup per CU and tier by default: the two tiers' workgroups
+// fit on a CU together (registers, LDS), and the prepare kernels of new
+// queries must still find room on the CUs while the serve kernels run.
 int default_workgroups(int wg) {
     if (wg > 0) return wg;
     int dev = 0, cus = 0;
@@ -552,29 +766,38 @@ void to_result(const Completion& c, uint64_t tag, int type, double t0, double t1
 }  // namespace
 
 struct bgpq_server {
-    int workgroups = 0;
+    int workgroups = 0;  // per tier
     unsigned qcap = 0, scap = 0;
     char* arena_base = nullptr;
     Arena arena;
-    // pinned host memory, one allocation
+    // pinned host memory, one allocation: descriptors and completions (by
+    // query index), each tier's slot queue and words, the closed flag
     void* host = nullptr;
     QueryDesc* desc = nullptr;
-    uint32_t* slots = nullptr;
-    long long* tail = nullptr;
-    int* closed = nullptr;
-    int* fault = nullptr;  // written by the kernel
-    int* live_wg = nullptr;  // workgroups in the kernel, written by it
     Completion* done = nullptr;
+    int* closed = nullptr;
     long long max_polls = kMaxRingPolls;
-    // device word
-    long long* claim = nullptr;
-    hipStream_t run = nullptr, prep = nullptr;
+    // device words of every tier: claim counter, ring mirror, poller flag
+    char* dev = nullptr;
+    // device: a query index's setup words (chunk counter, chunks done),
+    // 16 bytes apart, zero between queries
+    unsigned* qsync = nullptr;
+    // pinned host slots for knapsack item arrays until their setup is done
+    char* stage = nullptr;
+    std::vector<int> stage_free;
+    struct Tier {
+        uint32_t* slots = nullptr;
+        long long* tail = nullptr;
+        int* fault = nullptr;    // written by the kernel
+        int* live_wg = nullptr;  // workgroups in the kernel, written by it
+        long long posted = 0, slots_live = 0;
+        hipStream_t run = nullptr;
+    } tier[kTiers];
+    hipStream_t prep = nullptr;
     bool running = false;
-    long long posted = 0;
-    long long slots_live = 0;
     struct Live {
         uint64_t tag;
-        int type, team;
+        int type, team, tier, stage;
         size_t off, bytes;
         unsigned qi;
         uint32_t* labels;
@@ -585,6 +808,24 @@ struct bgpq_server {
     std::vector<Live> live;
     std::vector<char> qi_busy;
     unsigned next_qi = 0;
+
+    ServeRing ring(int t) const {
+        char* d = dev + 512 * t;
+        return ServeRing{desc,
+                         tier[t].slots,
+                         tier[t].tail,
+                         closed,
+                         done,
+                         reinterpret_cast<long long*>(d),
+                         reinterpret_cast<unsigned long long*>(d + 128),
+                         reinterpret_cast<int*>(d + 256),
+                         tier[t].fault,
+                         tier[t].live_wg,
+                         max_polls,
+                         qcap - 1,
+                         scap - 1};
+    }
+    int live_of(int t) const { return __atomic_load_n(tier[t].live_wg, __ATOMIC_ACQUIRE); }
 };
 
 extern "C" double bgpq_now(void) { return now_s(); }
@@ -607,11 +848,18 @@ extern "C" int bgpq_server_create(bgpq_server** out, int64_t hbm_budget, int wor
     s->workgroups = default_workgroups(workgroups);
     s->qcap = qcap;
     s->scap = qcap * 16;
-    const size_t hbytes = sizeof(QueryDesc) * qcap + sizeof(Completion) * qcap + 4ull * s->scap + 512;
+    // per tier: the slots, then a 512-byte block (tail, live count, fault:
+    // a 128-byte line each); the closed flag at the end
+    const size_t o_done = sizeof(QueryDesc) * qcap, o_tiers = o_done + sizeof(Completion) * qcap;
+    const size_t tier_bytes = 4ull * s->scap + 512, hbytes = o_tiers + kTiers * tier_bytes + 128;
+    const int stage_slots = (int)std::min<unsigned>(qcap, 1024);
     hipError_t e = hipHostMalloc(&s->host, hbytes, hipHostMallocCoherent | hipHostMallocMapped);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&s->stage, kStageBytes * stage_slots, hipHostMallocCoherent | hipHostMallocMapped);
     if (e == hipSuccess) e = hipMalloc((void**)&s->arena_base, (size_t)hbm_budget);
-    if (e == hipSuccess) e = hipMalloc((void**)&s->claim, 256);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->run, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->dev, 512 * kTiers);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->qsync, 16 * (size_t)qcap);
+    for (int t = 0; t < kTiers && e == hipSuccess; ++t) e = hipStreamCreateWithFlags(&s->tier[t].run, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->prep, hipStreamNonBlocking);
     if (e == hipSuccess) e = allow_serve_lds();
     if (e != hipSuccess) {
@@ -621,12 +869,16 @@ extern "C" int bgpq_server_create(bgpq_server** out, int64_t hbm_budget, int wor
     std::memset(s->host, 0, hbytes);
     char* h = (char*)s->host;
     s->desc = (QueryDesc*)h;
-    s->done = (Completion*)(h + sizeof(QueryDesc) * qcap);
-    s->slots = (uint32_t*)(h + sizeof(QueryDesc) * qcap + sizeof(Completion) * qcap);
-    s->tail = (long long*)(h + hbytes - 128);
-    s->closed = (int*)(h + hbytes - 64);
-    s->live_wg = (int*)(h + hbytes - 192);
-    s->fault = (int*)(h + hbytes - 256);
+    s->done = (Completion*)(h + o_done);
+    for (int t = 0; t < kTiers; ++t) {
+        char* b = h + o_tiers + t * tier_bytes;
+        s->tier[t].slots = (uint32_t*)b;
+        s->tier[t].tail = (long long*)(b + 4ull * s->scap);
+        s->tier[t].live_wg = (int*)(b + 4ull * s->scap + 128);
+        s->tier[t].fault = (int*)(b + 4ull * s->scap + 256);
+    }
+    s->closed = (int*)(h + hbytes - 128);
+    for (int i = stage_slots - 1; i >= 0; --i) s->stage_free.push_back(i);
     s->arena.init((size_t)hbm_budget);
     s->qi_busy.assign(qcap, 0);
     *out = s;
@@ -636,16 +888,18 @@ extern "C" int bgpq_server_create(bgpq_server** out, int64_t hbm_budget, int wor
 extern "C" int bgpq_server_start(bgpq_server* s) {
     if (!s) return bgpq_fail(BGPQ_EINVAL, "null server");
     if (s->running) return bgpq_fail(BGPQ_EINVAL, "server already running");
-    __atomic_store_n(s->tail, 0ll, __ATOMIC_RELEASE);
     __atomic_store_n(s->closed, 0, __ATOMIC_RELEASE);
-    s->posted = 0;
-    __atomic_store_n(s->live_wg, s->workgroups, __ATOMIC_RELEASE);
-    __atomic_store_n(s->fault, 0, __ATOMIC_RELEASE);
     (void)hipGetLastError();
-    hipLaunchKernelGGL(k_ring_reset, dim3(1), dim3(1), 0, s->run, s->claim, s->fault, s->live_wg, s->workgroups);
-    const ServeRing r{s->desc, s->slots,   s->tail,      s->closed,    s->done,
-                      s->claim, s->fault, s->live_wg, s->max_polls, s->qcap - 1, s->scap - 1};
-    hipLaunchKernelGGL(k_serve, dim3(s->workgroups), dim3(kServeNT), app_lds_bytes<1024>(), s->run, r);
+    // before any serve kernel runs: setup words of every query index zero
+    BGPQ_HIPCHK(hipMemset(s->qsync, 0, 16 * (size_t)s->qcap));
+    for (int t = 0; t < kTiers; ++t) {
+        bgpq_server::Tier& T = s->tier[t];
+        __atomic_store_n(T.tail, 0ll, __ATOMIC_RELEASE);
+        T.posted = 0;
+        __atomic_store_n(T.live_wg, s->workgroups, __ATOMIC_RELEASE);
+        __atomic_store_n(T.fault, 0, __ATOMIC_RELEASE);
+        launch_tier(t, s->workgroups, T.run, s->ring(t));
+    }
     BGPQ_HIPCHK(hipGetLastError());
     s->running = true;
     return BGPQ_OK;
@@ -660,61 +914,71 @@ extern "C" int bgpq_server_set_idle_limit(bgpq_server* s, int64_t polls) {
 
 extern "C" int bgpq_server_live(const bgpq_server* s) {
     if (!s) return bgpq_fail(BGPQ_EINVAL, "null server");
-    return s->running ? __atomic_load_n(s->live_wg, __ATOMIC_ACQUIRE) : 0;
+    if (!s->running) return 0;
+    int n = 0;
+    for (int t = 0; t < kTiers; ++t) n += s->live_of(t);
+    return n;
 }
 
 extern "C" int bgpq_server_submit(bgpq_server* s, const bgpq_query* q) {
     if (!s) return bgpq_fail(BGPQ_EINVAL, "null server");
     if (!s->running) return bgpq_fail(BGPQ_EINVAL, "server not running");
-    // no workgroup is left to take the query: refuse it rather than queue it
-    if (__atomic_load_n(s->live_wg, __ATOMIC_ACQUIRE) <= 0)
-        return bgpq_fail(BGPQ_EHEAP, "the serve kernel has no workgroup left (idle limit reached)");
     int rc = check_query(q);
     if (rc) return rc;
-    if (s->live.size() >= s->qcap || s->slots_live + q->team > (long long)s->scap) return BGPQ_EAGAIN;
+    const int t = tier_of(q->batch_size);
+    bgpq_server::Tier& T = s->tier[t];
+    // no workgroup of the query's tier is left to take it: refuse it rather
+    // than queue it
+    if (s->live_of(t) <= 0)
+        return bgpq_fail(BGPQ_EHEAP, "the serve kernel has no workgroup left (idle limit reached)");
+    if (s->live.size() >= s->qcap || T.slots_live + q->team > (long long)s->scap) return BGPQ_EAGAIN;
     const Layout L = layout_of(q);
     if (L.total > s->arena.size) return bgpq_fail(BGPQ_EINVAL, "query needs %zu bytes, budget is %zu", L.total, s->arena.size);
+    const bool knap = q->type == BGPQ_Q_KNAPSACK;
+    if (knap && s->stage_free.empty()) return BGPQ_EAGAIN;
     size_t off = 0;
     if (!s->arena.alloc(L.total, &off)) return BGPQ_EAGAIN;
     unsigned qi = s->next_qi;
     while (s->qi_busy[qi & (s->qcap - 1)]) ++qi;
     qi &= s->qcap - 1;
     s->next_qi = qi + 1;
+    const int stage = knap ? s->stage_free.back() : -1;
+    // no GPU call here: the query's team prepares its block (setup_query)
     QueryDesc d;
     FillJobs jobs;
-    rc = build_query(q, s->arena_base + off, L, s->prep, &d, &jobs);
-    if (rc == BGPQ_OK) {
-        (void)hipGetLastError();
-        launch_fills(jobs, s->prep, 256);
-        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->prep) != hipSuccess)
-            rc = bgpq_fail(BGPQ_EHIP, "prepare");
-    }
+    rc = build_query(q, s->arena_base + off, L, s->prep, &d, &jobs, true,
+                     knap ? s->stage + kStageBytes * (size_t)stage : nullptr);
     if (rc) {
         s->arena.release(off, L.total);
         return rc;
     }
+    if (knap) s->stage_free.pop_back();
+    d.sync = s->qsync + 4 * (size_t)qi;
     s->qi_busy[qi] = 1;
     std::memcpy(&s->desc[qi], &d, sizeof d);
     s->done[qi].done = 0;
-    for (int m = 0; m < q->team; ++m) s->slots[(s->posted + m) & (s->scap - 1)] = (qi << 8) | (uint32_t)m;
+    for (int m = 0; m < q->team; ++m) T.slots[(T.posted + m) & (s->scap - 1)] = (qi << 8) | (uint32_t)m;
     std::atomic_thread_fence(std::memory_order_release);
-    s->posted += q->team;
-    __atomic_store_n(s->tail, s->posted, __ATOMIC_RELEASE);
-    s->slots_live += q->team;
-    s->live.push_back(bgpq_server::Live{q->tag, q->type, q->team, off, L.total, qi, d.labels, L.n_labels,
+    T.posted += q->team;
+    __atomic_store_n(T.tail, T.posted, __ATOMIC_RELEASE);
+    T.slots_live += q->team;
+    s->live.push_back(bgpq_server::Live{q->tag, q->type, q->team, t, stage, off, L.total, qi, d.labels, L.n_labels,
                                         q->labels_out, now_s()});
     return BGPQ_OK;
 }
 
 extern "C" int bgpq_server_poll(bgpq_server* s, bgpq_result* out, int max_results) {
     if (!s) return bgpq_fail(BGPQ_EINVAL, "null server");
-    // read before the done flags: with no workgroup left, a query still
-    // not done below can never complete
-    const bool gone = s->running && __atomic_load_n(s->live_wg, __ATOMIC_ACQUIRE) <= 0;
+    // read before the done flags: with no workgroup of its tier left, a
+    // query still not done below can never complete
+    bool gone[kTiers];
+    for (int t = 0; t < kTiers; ++t) gone[t] = s->running && s->live_of(t) <= 0;
     int n = 0;
+    size_t stuck = 0;
     for (size_t i = 0; i < s->live.size() && n < max_results;) {
         bgpq_server::Live& L = s->live[i];
         if (!__atomic_load_n(&s->done[L.qi].done, __ATOMIC_ACQUIRE)) {
+            stuck += gone[L.tier];
             ++i;
             continue;
         }
@@ -728,14 +992,14 @@ extern "C" int bgpq_server_poll(bgpq_server* s, bgpq_result* out, int max_result
         if (out) to_result(c, L.tag, L.type, L.t_submit, t1, &out[n]);
         ++n;
         s->arena.release(L.off, L.bytes);
+        if (L.stage >= 0) s->stage_free.push_back(L.stage);
         s->qi_busy[L.qi] = 0;
-        s->slots_live -= L.team;
+        s->tier[L.tier].slots_live -= L.team;
         s->live[i] = s->live.back();
         s->live.pop_back();
     }
-    if (n == 0 && gone && !s->live.empty())
-        return bgpq_fail(BGPQ_EHEAP, "%zu queries cannot complete: the serve kernel has no workgroup left",
-                         s->live.size());
+    if (n == 0 && stuck)
+        return bgpq_fail(BGPQ_EHEAP, "%zu queries cannot complete: the serve kernel has no workgroup left", stuck);
     return n;
 }
 
@@ -747,11 +1011,15 @@ extern "C" int bgpq_server_close(bgpq_server* s) {
     if (!s) return bgpq_fail(BGPQ_EINVAL, "null server");
     if (!s->running) return BGPQ_OK;
     __atomic_store_n(s->closed, 1, __ATOMIC_RELEASE);
-    hipError_t e = hipStreamSynchronize(s->run);
+    hipError_t e = hipSuccess;
+    for (int t = 0; t < kTiers; ++t) {
+        const hipError_t et = hipStreamSynchronize(s->tier[t].run);
+        if (e == hipSuccess) e = et;
+    }
     s->running = false;
     if (e != hipSuccess) return bgpq_fail(BGPQ_EHIP, "serve kernel: %s", hipGetErrorString(e));
     // Workgroups that left at the idle limit are no fault; a posted query
-    // that the ended kernel never completed is.
+    // that the ended kernels never completed is.
     size_t lost = 0;
     for (const auto& L : s->live)
         if (!__atomic_load_n(&s->done[L.qi].done, __ATOMIC_ACQUIRE)) ++lost;
@@ -762,85 +1030,124 @@ extern "C" int bgpq_server_close(bgpq_server* s) {
 extern "C" void bgpq_server_destroy(bgpq_server* s) {
     if (!s) return;
     if (s->running) (void)bgpq_server_close(s);
-    if (s->run) (void)hipStreamDestroy(s->run);
+    for (int t = 0; t < kTiers; ++t)
+        if (s->tier[t].run) (void)hipStreamDestroy(s->tier[t].run);
     if (s->prep) (void)hipStreamDestroy(s->prep);
-    if (s->claim) (void)hipFree(s->claim);
+    if (s->dev) (void)hipFree(s->dev);
+    if (s->qsync) (void)hipFree(s->qsync);
+    if (s->stage) (void)hipHostFree(s->stage);
     if (s->arena_base) (void)hipFree(s->arena_base);
     if (s->host) (void)hipHostFree(s->host);
     delete s;
 }
 
 // ---------------------------------------------------------------- batch
-// All queries at once: the arena holds every query, the ring lives in
-// device memory fully posted and closed, and one captured graph runs
-// [fills of every query -> ring reset -> serve kernel]; replays start from
-// the same state.
+// All queries at once: the arena holds every query, each tier's ring lives
+// in device memory fully posted and closed, and one captured graph runs
+// [fills of every query -> (ring reset -> serve kernel) of each tier, the
+// tiers as parallel branches]; replays start from the same state.
 extern "C" int bgpq_batch_run(const bgpq_query* q, int n, int64_t hbm_budget, int workgroups, int repeats,
                               bgpq_result* out, float* ms_per_run) {
     if (!q || n < 1 || repeats < 1) return bgpq_fail(BGPQ_EINVAL, "batch: bad arguments");
     std::vector<Layout> lay(n);
     size_t total = 0;
-    long long slots = 0;
+    long long slots[kTiers] = {};
     for (int i = 0; i < n; ++i) {
         const int rc = check_query(&q[i]);
         if (rc) return rc;
         lay[i] = layout_of(&q[i]);
         total += align_up(lay[i].total, 256);
-        slots += q[i].team;
+        slots[tier_of(q[i].batch_size)] += q[i].team;
     }
     if ((int64_t)total > hbm_budget)
         return bgpq_fail(BGPQ_EINVAL, "batch needs %zu bytes, budget %lld", total, (long long)hbm_budget);
-    unsigned qcap = 1, scap = 1;
+    unsigned qcap = 1, scap[kTiers];
     while ((int)qcap < n) qcap <<= 1;
-    while ((long long)scap < slots) scap <<= 1;
     if (qcap > (1u << 24)) return bgpq_fail(BGPQ_EINVAL, "batch too large");
-    const size_t o_done = sizeof(QueryDesc) * qcap, o_slots = o_done + sizeof(Completion) * qcap;
-    const size_t o_tail = align_up(o_slots + 4ull * scap, 256), o_closed = o_tail + 64, o_claim = o_tail + 128,
-                 o_live = o_tail + 160, o_fault = o_tail + 192, rbytes = o_tail + 256;
+    // ring: descriptors, completions, then per tier its slots and a
+    // 1024-byte block of words (a 128-byte line each)
+    const size_t o_done = sizeof(QueryDesc) * qcap;
+    size_t o_slots[kTiers], o_words[kTiers], rbytes = o_done + sizeof(Completion) * qcap;
+    for (int t = 0; t < kTiers; ++t) {
+        scap[t] = 1;
+        while ((long long)scap[t] < slots[t]) scap[t] <<= 1;
+        o_slots[t] = align_up(rbytes, 256);
+        o_words[t] = align_up(o_slots[t] + 4ull * scap[t], 256);
+        rbytes = o_words[t] + 1024;
+    }
+    enum { W_TAIL = 0, W_CLOSED = 128, W_CLAIM = 256, W_MIRROR = 384, W_POLLER = 512, W_LIVE = 640, W_FAULT = 768 };
     char* arena = nullptr;
     char* ring = nullptr;
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr, s1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, fork = nullptr, join = nullptr;
     std::vector<QueryDesc> desc(qcap);
     std::vector<FillJobs> jobs(n);
-    std::vector<uint32_t> slot(scap, 0);
+    std::vector<uint32_t> slot[kTiers];
+    for (int t = 0; t < kTiers; ++t) slot[t].assign(scap[t], 0);
     std::vector<Completion> comp(qcap);
     int rc = BGPQ_OK;
     if (hipMalloc((void**)&arena, total) != hipSuccess || hipMalloc((void**)&ring, rbytes) != hipSuccess ||
-        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-        hipEventCreate(&e1) != hipSuccess || allow_serve_lds() != hipSuccess)
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s1, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess || allow_serve_lds() != hipSuccess)
         rc = bgpq_fail(BGPQ_EHIP, "batch allocation");
     size_t off = 0;
-    long long posted = 0;
+    long long posted[kTiers] = {};
     for (int i = 0; i < n && rc == BGPQ_OK; ++i) {
         rc = build_query(&q[i], arena + off, lay[i], s, &desc[i], &jobs[i]);
-        for (int m = 0; m < q[i].team; ++m) slot[posted++] = ((uint32_t)i << 8) | (uint32_t)m;
+        const int t = tier_of(q[i].batch_size);
+        for (int m = 0; m < q[i].team; ++m) slot[t][posted[t]++] = ((uint32_t)i << 8) | (uint32_t)m;
         off += align_up(lay[i].total, 256);
     }
-    const long long tail = posted;
     const int closed = 1;
     if (rc == BGPQ_OK &&
-        (hipMemcpyAsync(ring, desc.data(), sizeof(QueryDesc) * qcap, hipMemcpyHostToDevice, s) != hipSuccess ||
-         hipMemcpyAsync(ring + o_slots, slot.data(), 4ull * scap, hipMemcpyHostToDevice, s) != hipSuccess ||
-         hipMemcpyAsync(ring + o_tail, &tail, sizeof tail, hipMemcpyHostToDevice, s) != hipSuccess ||
-         hipMemcpyAsync(ring + o_closed, &closed, sizeof closed, hipMemcpyHostToDevice, s) != hipSuccess ||
-         hipStreamSynchronize(s) != hipSuccess))
+        hipMemcpyAsync(ring, desc.data(), sizeof(QueryDesc) * qcap, hipMemcpyHostToDevice, s) != hipSuccess)
         rc = bgpq_fail(BGPQ_EHIP, "batch ring upload");
+    for (int t = 0; t < kTiers && rc == BGPQ_OK; ++t) {
+        if (hipMemcpyAsync(ring + o_slots[t], slot[t].data(), 4ull * scap[t], hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpyAsync(ring + o_words[t] + W_TAIL, &posted[t], sizeof(long long), hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipMemcpyAsync(ring + o_words[t] + W_CLOSED, &closed, sizeof closed, hipMemcpyHostToDevice, s) != hipSuccess)
+            rc = bgpq_fail(BGPQ_EHIP, "batch ring upload");
+    }
+    if (rc == BGPQ_OK && hipStreamSynchronize(s) != hipSuccess) rc = bgpq_fail(BGPQ_EHIP, "batch ring upload");
+    auto ring_of = [&](int t) {
+        char* w = ring + o_words[t];
+        return ServeRing{(const QueryDesc*)ring,
+                         (const uint32_t*)(ring + o_slots[t]),
+                         (const long long*)(w + W_TAIL),
+                         (const int*)(w + W_CLOSED),
+                         (Completion*)(ring + o_done),
+                         (long long*)(w + W_CLAIM),
+                         (unsigned long long*)(w + W_MIRROR),
+                         (int*)(w + W_POLLER),
+                         (int*)(w + W_FAULT),
+                         (int*)(w + W_LIVE),
+                         kMaxRingPolls,
+                         qcap - 1,
+                         scap[t] - 1};
+    };
     if (rc == BGPQ_OK) {
         if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             rc = bgpq_fail(BGPQ_EHIP, "begin capture");
         } else {
             for (int i = 0; i < n; ++i) launch_fills(jobs[i], s, 64);
             const int wgs = default_workgroups(workgroups);
-            hipLaunchKernelGGL(k_ring_reset, dim3(1), dim3(1), 0, s, (long long*)(ring + o_claim),
-                               (int*)(ring + o_fault), (int*)(ring + o_live), wgs);
-            const ServeRing r{(const QueryDesc*)ring, (const uint32_t*)(ring + o_slots),
-                              (const long long*)(ring + o_tail), (const int*)(ring + o_closed),
-                              (Completion*)(ring + o_done), (long long*)(ring + o_claim), (int*)(ring + o_fault),
-                              (int*)(ring + o_live), kMaxRingPolls, qcap - 1, scap - 1};
-            hipLaunchKernelGGL(k_serve, dim3(wgs), dim3(kServeNT), app_lds_bytes<1024>(), s, r);
+            // tier 1 on a forked branch, joined before the capture ends
+            const bool both = posted[0] > 0 && posted[1] > 0;
+            if (both) {
+                (void)hipEventRecord(fork, s);
+                (void)hipStreamWaitEvent(s1, fork, 0);
+            }
+            if (posted[0] > 0) launch_tier(0, wgs, s, ring_of(0));
+            if (posted[1] > 0) launch_tier(1, wgs, both ? s1 : s, ring_of(1));
+            if (both) {
+                (void)hipEventRecord(join, s1);
+                (void)hipStreamWaitEvent(s, join, 0);
+            }
             if (hipStreamEndCapture(s, &graph) != hipSuccess || !graph) rc = bgpq_fail(BGPQ_EHIP, "end capture");
         }
     }
@@ -861,11 +1168,17 @@ extern "C" int bgpq_batch_run(const bgpq_query* q, int n, int64_t hbm_budget, in
             break;
         }
         total_ms += ms;
-        if (hipMemcpy(comp.data(), ring + o_done, sizeof(Completion) * qcap, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(&fault, ring + o_fault, sizeof fault, hipMemcpyDeviceToHost) != hipSuccess) {
+        if (hipMemcpy(comp.data(), ring + o_done, sizeof(Completion) * qcap, hipMemcpyDeviceToHost) != hipSuccess) {
             rc = bgpq_fail(BGPQ_EHIP, "batch results");
             break;
         }
+        for (int t = 0; t < kTiers && rc == BGPQ_OK; ++t) {
+            int f = 0;
+            if (posted[t] > 0 && hipMemcpy(&f, ring + o_words[t] + W_FAULT, sizeof f, hipMemcpyDeviceToHost) != hipSuccess)
+                rc = bgpq_fail(BGPQ_EHIP, "batch results");
+            fault |= f;
+        }
+        if (rc != BGPQ_OK) break;
         if (fault) {
             rc = bgpq_fail(BGPQ_EHEAP, "batch serve kernel fault");
             break;
@@ -897,6 +1210,9 @@ extern "C" int bgpq_batch_run(const bgpq_query* q, int n, int64_t hbm_budget, in
     if (graph) (void)hipGraphDestroy(graph);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (s1) (void)hipStreamDestroy(s1);
     if (s) (void)hipStreamDestroy(s);
     if (ring) (void)hipFree(ring);
     if (arena) (void)hipFree(arena);

@@ -1,8 +1,8 @@
-/* Batched and served heap solves: one persistent kernel runs many
- * independent queries (SSSP, A*, knapsack), each on its own device-resident
- * heap, solved by a team of workgroups.  New queries join while others run:
- * the host posts them into a ring in pinned host memory that the kernel's
- * idle workgroups poll.  Device memory for every query (heap, labels,
+/* Batched and served heap solves: persistent kernels run many independent
+ * queries (SSSP, A*, knapsack), each on its own device-resident heap,
+ * solved by a team of workgroups.  New queries join while others run: the
+ * host posts them into a ring in pinned host memory, which one idle
+ * workgroup at a time reads for the others.  Device memory for every query (heap, labels,
  * search state) comes from one arena whose size is the HBM budget; a query
  * that does not fit waits (BGPQ_EAGAIN) until completions free memory.
  *
@@ -12,7 +12,9 @@
  * applications/sssp_longfei_new/main.cu:96-102).
  *
  * Queries use the buitdd heap variant with u64 keys and a batch size of
- * 256, 512 or 1024; the server's workgroups have 256 threads.
+ * 256, 512 or 1024.  There are two serve kernels (tiers), each with its
+ * own slot queue: 256-thread workgroups for B = 256 and 512, 512-thread
+ * workgroups for B = 1024.
  */
 #ifndef BGPQ_SERVE_H
 #define BGPQ_SERVE_H
@@ -78,26 +80,26 @@ double bgpq_now(void);
 /* HBM footprint of a query in the arena (bytes), or a negative status. */
 int64_t bgpq_query_bytes(const bgpq_query* q);
 
-/* hbm_budget: arena bytes; workgroups: size of the persistent kernel
- * (0: one per CU); queue_slots: most queries in flight (0: 4096).
+/* hbm_budget: arena bytes; workgroups: size of each tier's persistent
+ * kernel (0: one per CU); queue_slots: most queries in flight (0: 4096).
  * Inputs the queries point to (CSR graphs, obstacle maps) must be in device
  * memory before bgpq_server_start: the kernel reads them with plain loads. */
 int bgpq_server_create(bgpq_server** out, int64_t hbm_budget, int workgroups, int queue_slots);
 void bgpq_server_destroy(bgpq_server* s);
-/* Idle polls (each about 3.4 us) a workgroup of the persistent kernel makes
+/* Idle polls (each about 3.4 us) a workgroup of a persistent kernel makes
  * while nothing is posted before it leaves the kernel; 0 restores the
  * default (about a minute).  Set before bgpq_server_start. */
 int bgpq_server_set_idle_limit(bgpq_server* s, int64_t polls);
-/* Launch the persistent kernel (on its own stream). */
+/* Launch the persistent kernels (each tier on its own stream). */
 int bgpq_server_start(bgpq_server* s);
-/* Workgroups of the running kernel that have not left it. */
+/* Workgroups of the running kernels (both tiers) that have not left them. */
 int bgpq_server_live(const bgpq_server* s);
 /* Prepare the query's state in the arena and post it.  Returns BGPQ_OK,
  * BGPQ_EAGAIN when the arena or the queue is full now, or an error
- * (BGPQ_EHEAP once every workgroup has left the kernel at the idle limit:
- * nothing could take the query).  A workgroup takes a member slot only
- * once it is posted, so a posted query is solved while any workgroup is
- * left. */
+ * (BGPQ_EHEAP once every workgroup of the query's tier has left at the
+ * idle limit: nothing could take the query).  A workgroup takes a member
+ * slot only once it is posted, so a posted query is solved while any
+ * workgroup of its tier is left. */
 int bgpq_server_submit(bgpq_server* s, const bgpq_query* q);
 /* Collect up to max_results completed queries (their arena memory is
  * freed); returns how many were written, or BGPQ_EHEAP when none completed
