@@ -17,6 +17,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "bgpq.h"
 
@@ -83,6 +84,23 @@ class Heap {
         return (int)v;
     }
     bool isEmpty() { return check(bgpq_is_empty(h_)) == 1; }
+    // printHeap() -- heap/buitdd/heap.cuh:175-194: the partial buffer, then
+    // every batch in use, each line tagged with its slot's state word
+    void printHeap() {
+        const size_t keys = (size_t)(batchNum_ + 1) * batchSize_;
+        std::vector<K> items(keys);
+        std::vector<int> status(batchNum_ + 1);
+        int ctr[8] = {};
+        check(bgpq_snapshot(h_, items.data(), status.data(), ctr));
+        // "batch partial <size>_<state>:" and "batch <slot>_<state>:", keys after
+        auto line = [&](const char* what, int label, int slot, int n) {
+            std::printf("%s %d_%d:", what, label, status[slot]);
+            for (int i = 0; i < n; ++i) std::printf(" %llu", (unsigned long long)items[(size_t)slot * batchSize_ + i]);
+            std::printf("\n");
+        };
+        line("batch partial", ctr[1], 0, ctr[1]);
+        for (int i = 1; i <= ctr[0]; ++i) line("batch", i, i, batchSize_);
+    }
 
     // The harness kernels.  insertKernel: n keys from d_items in calls of
     // `chunk` keys; deleteKernel: `calls` deleteRoot/deleteUpdate pairs,

@@ -37,7 +37,27 @@ static double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// `heap_test print`: printHeap of a small buitdd heap (B = 64) holding
+// keys 0..199 (three full batches and a partial buffer of 8).
+static int print_small() {
+    try {
+        Heap<uint32_t> h(8, 64, UINT32_MAX, Variant::buitdd);
+        std::vector<uint32_t> keys(200);
+        for (uint32_t i = 0; i < 200; ++i) keys[i] = (i * 7919u) % 200u;
+        bgpq::DeviceArray<uint32_t> d(200);
+        d.upload(keys.data(), 200);
+        h.insertKernel(d.get(), 200, 50, 1);
+        bgpq::check(bgpq_sync());
+        h.printHeap();
+    } catch (const bgpq::Error& e) {
+        std::printf("%s\n", e.what());
+        return -1;
+    }
+    return 0;
+}
+
 int main(int argc, char* argv[]) {
+    if (argc == 2 && std::string(argv[1]) == "print") return print_small();
     if (argc < 3) {
         std::printf("%s [batch size] [array num] [variant] [chunk] [seed]\n", argv[0]);
         return -1;
