@@ -82,6 +82,9 @@ struct QueryDesc {
     FillJobs setup;
     int32_t setup_chunks, pad2;
     unsigned* sync;
+    // served queries that want their labels back: pinned host memory the
+    // last member copies them to (the host copies on from there)
+    uint32_t* labels_host;
 };
 static_assert(sizeof(QueryDesc) % 8 == 0 && sizeof(QueryDesc) / 8 <= 256, "descriptor words");
 
@@ -266,6 +269,10 @@ __device__ void finish(const QueryDesc& d, Completion* c) {
             red[0][t / 64] = dig;
             red[1][t / 64] = reach;
         }
+    }
+    if (d.labels_host) {
+        for (long long v = t; v < d.n_labels; v += NT) d.labels_host[v] = u32_load(d.labels + v);
+        __threadfence_system();  // before the completion below is released
     }
     __syncthreads();
     if (t == 0) {
@@ -785,6 +792,10 @@ struct bgpq_server {
     // pinned host slots for knapsack item arrays until their setup is done
     char* stage = nullptr;
     std::vector<int> stage_free;
+    // pinned host buffers the kernels write returned labels to (bytes,
+    // pointer): kept for reuse until the server is destroyed
+    std::vector<std::pair<size_t, void*>> lab_free;
+    std::vector<void*> lab_all;
     struct Tier {
         uint32_t* slots = nullptr;
         long long* tail = nullptr;
@@ -803,6 +814,8 @@ struct bgpq_server {
         uint32_t* labels;
         long long n_labels;
         uint32_t* labels_out;
+        void* lab_host;  // pinned copy of the labels, or null
+        size_t lab_bytes;
         double t_submit;
     };
     std::vector<Live> live;
@@ -952,8 +965,32 @@ extern "C" int bgpq_server_submit(bgpq_server* s, const bgpq_query* q) {
         s->arena.release(off, L.total);
         return rc;
     }
+    // labels requested: a pinned buffer the last member writes them to (the
+    // runtime's device-to-host copy can wait for a free CU, which the
+    // resident serve kernels may not leave)
+    void* lab = nullptr;
+    size_t lab_bytes = 0;
+    if (q->labels_out && L.n_labels > 0) {
+        lab_bytes = 4 * (size_t)L.n_labels;
+        for (size_t i = 0; i < s->lab_free.size(); ++i) {
+            if (s->lab_free[i].first >= lab_bytes) {
+                lab = s->lab_free[i].second;
+                lab_bytes = s->lab_free[i].first;
+                s->lab_free.erase(s->lab_free.begin() + (long)i);
+                break;
+            }
+        }
+        if (!lab) {
+            if (hipHostMalloc(&lab, lab_bytes, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess) {
+                s->arena.release(off, L.total);
+                return bgpq_fail(BGPQ_EHIP, "pinned memory for %zu label bytes", lab_bytes);
+            }
+            s->lab_all.push_back(lab);
+        }
+    }
     if (knap) s->stage_free.pop_back();
     d.sync = s->qsync + 4 * (size_t)qi;
+    d.labels_host = (uint32_t*)lab;
     s->qi_busy[qi] = 1;
     std::memcpy(&s->desc[qi], &d, sizeof d);
     s->done[qi].done = 0;
@@ -963,7 +1000,7 @@ extern "C" int bgpq_server_submit(bgpq_server* s, const bgpq_query* q) {
     __atomic_store_n(T.tail, T.posted, __ATOMIC_RELEASE);
     T.slots_live += q->team;
     s->live.push_back(bgpq_server::Live{q->tag, q->type, q->team, t, stage, off, L.total, qi, d.labels, L.n_labels,
-                                        q->labels_out, now_s()});
+                                        q->labels_out, lab, lab_bytes, now_s()});
     return BGPQ_OK;
 }
 
@@ -985,9 +1022,9 @@ extern "C" int bgpq_server_poll(bgpq_server* s, bgpq_result* out, int max_result
         const double t1 = now_s();
         Completion c;
         std::memcpy(&c, &s->done[L.qi], sizeof c);
-        if (L.labels_out && L.n_labels > 0) {
-            BGPQ_HIPCHK(hipMemcpyAsync(L.labels_out, L.labels, 4 * (size_t)L.n_labels, hipMemcpyDeviceToHost, s->prep));
-            BGPQ_HIPCHK(hipStreamSynchronize(s->prep));
+        if (L.lab_host) {
+            std::memcpy(L.labels_out, L.lab_host, 4 * (size_t)L.n_labels);
+            s->lab_free.emplace_back(L.lab_bytes, L.lab_host);
         }
         if (out) to_result(c, L.tag, L.type, L.t_submit, t1, &out[n]);
         ++n;
@@ -1036,6 +1073,7 @@ extern "C" void bgpq_server_destroy(bgpq_server* s) {
     if (s->dev) (void)hipFree(s->dev);
     if (s->qsync) (void)hipFree(s->qsync);
     if (s->stage) (void)hipHostFree(s->stage);
+    for (void* p : s->lab_all) (void)hipHostFree(p);
     if (s->arena_base) (void)hipFree(s->arena_base);
     if (s->host) (void)hipHostFree(s->host);
     delete s;

@@ -174,3 +174,22 @@ def test_server_idle_limit_refuses_after_workgroups_leave(gpu):
         srv.close()
     finally:
         srv.destroy()
+
+
+def test_server_labels_out_while_serving(gpu):
+    """A served SSSP query's labels come back through labels_out while the
+    serve kernels keep running at full size (one workgroup per CU and
+    tier), promptly, and equal Dijkstra's distances."""
+    print("[case] serve labels_out", flush=True)
+    rp, col, w = A.rmat_csr(12, 16, 12)
+    g = A.Graph(rp, col, w)
+    dist = O.dijkstra(rp, col, w, 5)
+    with S.Server(1 * GiB) as srv:
+        for tag, (B, team) in enumerate(((1024, 4), (256, 2))):
+            out = np.zeros(g.n, np.uint32)
+            t0 = time.time()
+            assert srv.submit(S.sssp_query(g, 5, batch_size=B, team=team, tag=tag, labels_out=out))
+            got = _drain(srv, 1, timeout=20.0)
+            assert time.time() - t0 < 5.0, "a served query with labels_out took too long"
+            assert got[tag]["status"] == 0
+            assert np.array_equal(out, dist)
