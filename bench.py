@@ -461,7 +461,7 @@ def main():
     ap.add_argument("--sweep-sets", default="u32:random,u32:descending,u64:random",
                     help="key sets of the sweep, key:order pairs")
     ap.add_argument("--calibrate", action="store_true",
-                    help="also run a plain 1 GiB device copy (HBM counter calibration under rocprofv3)")
+                    help="also write and read back 1 GiB with plain kernels (HBM counter calibration under rocprofv3)")
     ap.add_argument("--queries", type=int, default=2000, help="--app mixed: queries in the stream")
     ap.add_argument("--line-queries", type=int, default=1000,
                     help="queries of the stratified mixed stream in the default line (0: leave it out)")
@@ -599,15 +599,18 @@ def main():
         m.pop("size_class_table", None)
         line["mixed"] = m
     if args.calibrate:
-        # a plain elementwise kernel of known traffic (reads and writes 1 GiB)
-        # to scale the FETCH_SIZE / WRITE_SIZE counters
-        import torch
+        # kernels of known traffic to scale the FETCH_SIZE / WRITE_SIZE
+        # counters: the key generator writes 1 GiB (k_gen_u32), the digest
+        # kernel reads the same 1 GiB back (k_digest)
         nb = 1 << 30
-        src = torch.ones(nb // 4, dtype=torch.int32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.mul(src, 3, out=dst)
-        torch.cuda.synchronize()
-        line["calibration_copy_bytes"] = nb
+        cal = DeviceBuffer(nb)
+        dig = DeviceBuffer(16)
+        check(lib().bgpq_gen_keys(0, 1, nb // 4, cal.ptr, None))
+        check(lib().bgpq_label_digest(cal.ptr, nb // 4, dig.ptr, None))
+        G.sync()
+        cal.free()
+        dig.free()
+        line["calibration_bytes"] = nb
     if cpu_legs is not None:
         print("[bench] waiting for the CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = finish_cpu_baseline(cpu_legs, args.key, r["keys_per_s"])
