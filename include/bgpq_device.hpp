@@ -797,10 +797,10 @@ struct DevHeap {
     //    own and released with the next level's publish ("pending"): no other
     //    workgroup can want it before the node below is released, which
     //    happens at that same publish;
-    //  - bu: the locks of oth's children are requested as soon as the descent
-    //    is known (and given back if the sift stops here), and when both are
-    //    granted their batches are loaded while the larger half is computed,
-    //    so the next level starts with its children in LDS.
+    //  - bu: the locks of oth's children are requested without waiting, just
+    //    before the node's store (and given back if the sift stops here), and
+    //    when both are granted their batches are loaded while the larger half
+    //    is computed, so the next level starts with its children in LDS.
     // Every wait is for a node deeper than every node held, as before.
     __device__ void sift_down(K* X0, K* X1, K* X2) const {
         constexpr bool kSplit = B >= kSplitFrom;
@@ -905,21 +905,22 @@ struct DevHeap {
             const K kmin_ = kmin(X1[0], X2[0]);  // the children's smallest key
             const bool stop = (VAR == VAR_BUITDD) ? (cmax < kmin_) : !(kmin_ < cmax);
             if constexpr (kSplit) {
-                if constexpr (kAhead) {
-                    if (oth_in) pre = lock_ahead(oth);
-                }
                 HalfMerge<K, B, NT, false> lo;
                 lo.run(X1, X2);
+                BGPQ_EVENT(30);
                 if (threadIdx.x == NT - 1) *key_slot<K>() = lo.last();
-                if constexpr (kAhead) {
-                    if (threadIdx.x < 2) bc[BC_C + threadIdx.x] = pre;
-                }
                 // the larger child stored one level up is released at the
                 // barrier below: every wave waits for its own stores to it
                 // first, before it issues the node's (issued a whole half
                 // merge ago, they have nearly always landed by now), so the
                 // next deleter can load it while it waits for this node
                 if (pend) wait_stores();
+                BGPQ_EVENT(31);
+                // the lock requests ahead go out after that wait (it would wait
+                // for their atomics too); their answers are read at the publish
+                if constexpr (kAhead) {
+                    if (oth_in) pre = lock_ahead(oth);
+                }
                 // the node's new batch in the usual case (below), stored
                 // before the barrier; the rare paths overwrite it
                 if (oth_in) lo.put(ToGlobal<K>{node(cur)});
@@ -932,6 +933,9 @@ struct DevHeap {
                 if (!(cmin < *key_slot<K>()) && oth_in) {
                     // candidate >= all of the small half: the node takes it
                     // (already stored above)
+                    if constexpr (kAhead) {
+                        if (threadIdx.x < 2) bc[BC_C + threadIdx.x] = pre;
+                    }
                     publish();
                     if (threadIdx.x == 0) {
                         release_st(cur, BU ? cur_prev : AVAIL);
@@ -953,10 +957,12 @@ struct DevHeap {
                     }
                     HalfMerge<K, B, NT, true> hi;
                     hi.run(X1, X2);
+                    BGPQ_EVENT(32);
                     if (tgt_in) hi.put(ToGlobal<K>{node(tgt)});
                     pend = tgt_in ? tgt : 0;
                     pend_word = BU ? tgt_prev : AVAIL;
                     __syncthreads();  // X1 and X2 are read
+                    BGPQ_EVENT(33);
                     if (fetch) {
                         fl.commit(X1);
                         fr.commit(X2);
