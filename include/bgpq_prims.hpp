@@ -52,6 +52,20 @@ __device__ __forceinline__ uint4 gload16(__amdgpu_buffer_rsrc_t r, int byte_off)
     const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, kSC1);
     return make_uint4(v.x, v.y, v.z, v.w);
 }
+// One key, device-coherent, at lane_off (per lane) + row_off (a constant)
+// bytes from the resource's base.
+template <typename K>
+__device__ __forceinline__ void gstore_row(__amdgpu_buffer_rsrc_t r, int lane_off, int row_off, K v) {
+    if constexpr (sizeof(K) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, r, lane_off, row_off, kSC1);
+    } else {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        u32x2 q;
+        q.x = (uint32_t)v;
+        q.y = (uint32_t)((uint64_t)v >> 32);
+        __builtin_amdgcn_raw_buffer_store_b64(q, r, lane_off, row_off, kSC1);
+    }
+}
 __device__ __forceinline__ void gstore16(__amdgpu_buffer_rsrc_t r, int byte_off, uint4 q) {
     u32x4 v;
     v.x = q.x; v.y = q.y; v.z = q.z; v.w = q.w;
@@ -877,45 +891,13 @@ struct GroupMerge {
     }
 };
 
-// 4x4 transpose inside each quad of lanes: lane r of a quad holds
-// M[r][c] = v[4m+c] (for each block m of 4 registers) and ends up with
-// M[c][r], c = 0..3.  Two butterfly steps, each a DPP quad_perm exchange and
-// a select per register.
-template <int X>
-__device__ __forceinline__ uint32_t quad_xor(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, X == 1 ? 0xB1 : 0x4E, 0xF, 0xF, false);
-}
-template <int X>
-__device__ __forceinline__ uint64_t quad_xor(uint64_t v) {
-    return ((uint64_t)quad_xor<X>((uint32_t)(v >> 32)) << 32) | quad_xor<X>((uint32_t)v);
-}
-template <typename K, int L>
-__device__ __forceinline__ void quad_transpose(K (&v)[L]) {
-    static_assert(L % 4 == 0, "quad transpose");
-    const int r = (int)__lane_id() & 3;
-#pragma unroll
-    for (int m = 0; m < L; m += 4) {
-        // step 1: 2x2 blocks, rows/columns differing in bit 0
-#pragma unroll
-        for (int c = 0; c < 4; c += 2) {
-            const K t0 = quad_xor<1>(v[m + c + 1]), t1 = quad_xor<1>(v[m + c]);
-            v[m + c] = (r & 1) ? t0 : v[m + c];
-            v[m + c + 1] = (r & 1) ? v[m + c + 1] : t1;
-        }
-        // step 2: rows/columns differing in bit 1
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const K t0 = quad_xor<2>(v[m + c + 2]), t1 = quad_xor<2>(v[m + c]);
-            v[m + c] = (r & 2) ? t0 : v[m + c];
-            v[m + c + 2] = (r & 2) ? v[m + c + 2] : t1;
-        }
-    }
-}
-
 // Store the group layout (lane j of a G-lane group holds outputs off + s*G + j).
-// To global memory with L % 4 == 0, a quad transpose first gives lane
-// 4q + r the four consecutive keys off + (4m + r)*G + 4q .. +3 of each block
-// m, which go out as 16-byte stores (u32; two for u64) instead of 4-byte ones.
+// A row s of a group is one 128-byte line (G keys of 128/G bytes), so plain
+// key-sized stores, one per row, already write whole lines; the row's offset
+// is a constant in the instruction.  (A 4x4 transpose inside each quad of
+// lanes into 16-byte runs cost about 32 VALU operations per lane per half
+// merge; without it the heap sort of 2^26 u32 keys measured 8% faster at
+// B = 8192 and 5% at B = 4096, and the same at B <= 2048.)
 template <typename K, int L, int G, typename D>
 __device__ __forceinline__ void put_strided(const D& dst, int off, int j, const K (&v)[L]) {
     if constexpr (std::is_same<D, ToNone>::value) {
@@ -923,29 +905,12 @@ __device__ __forceinline__ void put_strided(const D& dst, int off, int j, const 
     } else if constexpr (std::is_same<D, ToLds<K>>::value) {
 #pragma unroll
         for (int s = 0; s < L; ++s) dst.p[off + s * G + j] = v[s];
-    } else if constexpr (L % 4 == 0 && G % 4 == 0) {
-        K w[L];
-#pragma unroll
-        for (int s = 0; s < L; ++s) w[s] = v[s];
-        quad_transpose<K, L>(w);
-        const int r = j & 3, q4 = j & ~3;
-        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(dst.p);
-#pragma unroll
-        for (int m = 0; m < L; m += 4) {
-            const int e = off + (m + r) * G + q4;  // first key of this lane's run
-            if constexpr (sizeof(K) == 4) {
-                gstore16(rs, e * 4, make_uint4((uint32_t)w[m], (uint32_t)w[m + 1], (uint32_t)w[m + 2],
-                                              (uint32_t)w[m + 3]));
-            } else {
-                gstore16(rs, e * 8, make_uint4((uint32_t)w[m], (uint32_t)((uint64_t)w[m] >> 32),
-                                              (uint32_t)w[m + 1], (uint32_t)((uint64_t)w[m + 1] >> 32)));
-                gstore16(rs, e * 8 + 16, make_uint4((uint32_t)w[m + 2], (uint32_t)((uint64_t)w[m + 2] >> 32),
-                                                   (uint32_t)w[m + 3], (uint32_t)((uint64_t)w[m + 3] >> 32)));
-            }
-        }
     } else {
+        static_assert(G * sizeof(K) == 128, "a group row is one 128-byte line");
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(dst.p);
+        const int lane_off = (off + j) * (int)sizeof(K);
 #pragma unroll
-        for (int s = 0; s < L; ++s) gstore_key(dst.p + off + s * G + j, v[s]);
+        for (int s = 0; s < L; ++s) gstore_row<K>(rs, lane_off, s * G * (int)sizeof(K), v[s]);
     }
 }
 
