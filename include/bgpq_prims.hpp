@@ -817,15 +817,25 @@ struct GroupMerge {
         // probes of a group fall on G distinct banks in both inputs and
         // each round leaves an interval of at most s; with n <= G + 1 the
         // probes are lo, lo+1, ... and at most one more round follows.
+        // Branch-free rounds: both probe loads at the clamped index every
+        // time (a short-circuit would make them a branch with its own exec
+        // mask), the group's count from its 32-bit half of the ballot.
         while (lo < hi) {
             const int n = hi - lo;
             const int st = ((n + G) / (G + 1)) | 1;
             const int q = lo + (j + 1) * st - 1;
-            const bool before = q < hi && !(b[D - 1 - kmin(q, hi - 1)] < a[kmin(q, hi - 1)]);
-            const uint64_t m = __ballot(before);
-            const int c = __popcll((m >> shift) & ((1ull << G) - 1));
+            const int qc = kmin(q, hi - 1);
+            const bool before = (q < hi) & !(b[D - 1 - qc] < a[qc]);
+            const uint64_t m = __builtin_amdgcn_ballot_w64(before);
+            int c;
+            if constexpr (G == 32) {
+                const int c_lo = __popc((uint32_t)m), c_hi = __popc((uint32_t)(m >> 32));
+                c = shift ? c_hi : c_lo;
+            } else {
+                c = __popcll((m >> shift) & ((1ull << G) - 1));
+            }
             const int nlo = lo + c * st;
-            const int nhi = (c < G && lo + (c + 1) * st - 1 < hi) ? lo + (c + 1) * st - 1 : hi;
+            const int nhi = c < G ? kmin(nlo + st - 1, hi) : hi;
             lo = nlo;
             hi = nhi;
         }
