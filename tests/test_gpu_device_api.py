@@ -96,7 +96,7 @@ def test_checker_reports_planted_corruption(gpu, variant):
         _write(v.items, 4 * slot_key, np.array([value], np.uint32))
         return old
 
-    def verdicts():
+    def outcomes():
         ok_g = h.checkInsertHeap()
         ok_m, why_m = m.check()
         return ok_g, h.last_check_message, ok_m, why_m
@@ -118,7 +118,7 @@ def test_checker_reports_planted_corruption(gpu, variant):
             cases.append(("partial buffer not ascending", [(3, int(mi[4])), (4, int(mi[3]))]))
     for name, edits in cases:
         olds = [both(k, val) for k, val in edits]
-        ok_g, why_g, ok_m, why_m = verdicts()
+        ok_g, why_g, ok_m, why_m = outcomes()
         assert not ok_g and not ok_m, name
         assert why_g == why_m, (name, why_g, why_m)
         for (k, _), old in zip(reversed(edits), reversed(olds)):
