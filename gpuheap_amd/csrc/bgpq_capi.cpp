@@ -450,7 +450,7 @@ int bgpq_test_merge(int key_type, int batch_size, const void* d_a, const void* d
                     int batches, int algo) {
     const int bi = batch_index(batch_size);
     const LaunchRow* rows = rows_for(key_type, 0);
-    if (bi < 0 || !rows || batches < 1 || rows[bi].batch_size == 0 || algo < 0 || algo > 5)
+    if (bi < 0 || !rows || batches < 1 || rows[bi].batch_size == 0 || algo < 0 || algo > 7)
         return fail(BGPQ_EINVAL, "bad primitive test arguments");
     hipError_t e = rows[bi].merge(d_a, d_b, d_small, d_large, batches, algo, nullptr);
     if (e != hipSuccess) return fail(BGPQ_EHIP, "merge launch: %s", hipGetErrorString(e));

@@ -70,7 +70,8 @@ __global__ __launch_bounds__(threads_for(B)) void k_prim_sort(const K* in, const
 
 // algo 0: the register bitonic merge, 1: the LDS merge path, 2: the group
 // merge, 3: the serial merge path, 4: the two halves one after the other,
-// 5: the lean merge path of the delete's rare root step
+// 5: the lean merge path of the delete's rare root step, 6: the two halves
+// with copy windows (the insertion's), 7: the group merge with copy windows
 template <typename K, int B, int ALGO>
 __global__ __launch_bounds__(threads_for(B)) void k_prim_merge(const K* a, const K* b, K* small, K* large) {
     constexpr int NT = threads_for(B);
@@ -86,12 +87,14 @@ __global__ __launch_bounds__(threads_for(B)) void k_prim_merge(const K* a, const
     else if constexpr (ALGO == 2) wg_merge_grp<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
     else if constexpr (ALGO == 3) wg_merge_ser<K, B, NT>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
     else if constexpr (ALGO == 5) wg_merge_lean<K, B, NT>(X0, X1, small + o, large + o);
+    else if constexpr (ALGO == 7) wg_merge_grp<K, B, NT, true>(X0, X1, ToGlobal<K>{small + o}, ToGlobal<K>{large + o});
     else {
         // the two halves one after the other, as the heap's sift steps take them
-        HalfMerge<K, B, NT, false> lo;
+        constexpr bool CP = ALGO == 6;
+        HalfMerge<K, B, NT, false, CP> lo;
         lo.run(X0, X1);
         lo.put(ToGlobal<K>{small + o});
-        HalfMerge<K, B, NT, true> hi;
+        HalfMerge<K, B, NT, true, CP> hi;
         hi.run(X0, X1);
         hi.put(ToGlobal<K>{large + o});
     }
@@ -213,7 +216,9 @@ struct Launch {
             case 2: return merge_as<2>(a, b, small, large, batches, s);
             case 3: return merge_as<3>(a, b, small, large, batches, s);
             case 4: return merge_as<4>(a, b, small, large, batches, s);
-            default: return merge_as<5>(a, b, small, large, batches, s);
+            case 5: return merge_as<5>(a, b, small, large, batches, s);
+            case 6: return merge_as<6>(a, b, small, large, batches, s);
+            default: return merge_as<7>(a, b, small, large, batches, s);
         }
     }
     static LaunchRow row() { return LaunchRow{B, &sort, &merge, &insert, &del, &conc}; }

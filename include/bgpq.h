@@ -146,7 +146,9 @@ int bgpq_default_grid(int batch_size);
  *   2 the group merge, 3 the serial branch-free merge path (the heap picks
  *   one of them by batch size), 4 the two halves computed one after the
  *   other from the same inputs, as the heap's sift steps do, 5 the
- *   low-register merge path of the delete's rare root step. */
+ *   low-register merge path of the delete's rare root step, 6 the two
+ *   halves and 7 the group merge with copy windows (a window that takes all
+ *   its keys from one input is copied), as an insertion takes them. */
 int bgpq_test_sort(int key_type, int batch_size, const void* d_in, const int* d_sizes, void* d_out,
                    uint64_t pad, int batches);
 int bgpq_test_merge(int key_type, int batch_size, const void* d_a, const void* d_b, void* d_small, void* d_large,

@@ -14,7 +14,7 @@
 // buffer's size) and with the same smemOffset.  `smemOffset` counts ints
 // into the kernel's dynamic LDS (`extern __shared__ int[]`), exactly as in
 // the reference; it must be a multiple of 4.  LDS needed past the offset:
-// insertion 2*B*sizeof(K) (3*B*sizeof(K) for B >= 2048), deleteUpdate
+// insertion 2*B*sizeof(K) (3*B*sizeof(K) for B >= 1024), deleteUpdate
 // 3*B*sizeof(K).
 //
 // A kernel written outside this library gets the heap's storage from the
@@ -137,7 +137,7 @@ __device__ __forceinline__ K* key_slot() {
 }
 
 template <typename K, int B>
-constexpr int insert_lds_bytes() { return (B >= 2048 ? 3 : 2) * B * (int)sizeof(K); }
+constexpr int insert_lds_bytes() { return (B >= 1024 ? 3 : 2) * B * (int)sizeof(K); }
 template <typename K, int B>
 constexpr int delete_lds_bytes() { return 3 * B * (int)sizeof(K); }
 
@@ -675,11 +675,12 @@ struct DevHeap {
     // 1.26e8 with one bitonic merge per step; B = 512 and 256 are 1% and 4%
     // slower split, where a half merge costs about as much as a whole one.
     static constexpr int kSplitFrom = 1024;
-    // bu insertion walk for B >= 2048: two half merges per step, the node's
-    // batch stays in LDS between steps (a third buffer) and the parent's load
-    // overlaps the node's lock (at B = 1024 one bitonic merge per step and
-    // the plain walk measured 4% faster)
-    static constexpr bool kSplitWalk = BU && B >= 2048;
+    // bu insertion walk for B >= 1024: two half merges per step, with copy
+    // windows, the node's batch stays in LDS between steps (a third buffer)
+    // and the parent's load overlaps the node's lock.  At B = 1024 this
+    // measured 2% faster than one bitonic merge per step and the plain walk
+    // once the copy windows came in (4% slower without them).
+    static constexpr bool kSplitWalk = BU && B >= 1024;
 
     // bu: lanes 0 and 1 request the locks of the two children of `nxt`
     // without waiting; the state word found (INUSE if another workgroup holds
@@ -1283,18 +1284,18 @@ struct DevHeap {
                         if (threadIdx.x == 0) release_st(par, par_word);
                         l2g_batch<K, B, NT>(node(cur), P);
                     } else if constexpr (!kSplitWalk) {
-                        // B < 2048: one full merge is cheaper than two halves here
+                        // B < 1024: one full merge is cheaper than two halves here
                         wg_merge<K, B, NT>(C, P, ToGlobal<K>{node(par)}, ToGlobal<K>{node(cur)});
                         publish();
                         if (threadIdx.x == 0) release_st(par, par_word);
                     } else {
-                        HalfMerge<K, B, NT, false> lo;
+                        HalfMerge<K, B, NT, false, true> lo;
                         lo.run(C, P);
                         lo.put(ToGlobal<K>{node(par)});
                         lo.put(ToLds<K>{S});
                         publish();
                         if (threadIdx.x == 0) release_st(par, par_word);
-                        HalfMerge<K, B, NT, true> hi;
+                        HalfMerge<K, B, NT, true, true> hi;
                         hi.run(C, P);
                         hi.put(ToGlobal<K>{node(cur)});
                         K* t = C;  // the parent's new batch is S
@@ -1350,7 +1351,7 @@ struct DevHeap {
                     l2g_batch<K, B, NT>(node(cur), X0);
                     l2l_batch<K, B, NT>(X0, X1);
                 } else if (X0[0] < X1[B - 1]) {
-                    wg_merge<K, B, NT>(X0, X1, ToGlobal<K>{node(cur)}, ToLds<K>{X0});
+                    wg_merge<K, B, NT, (B >= 4096)>(X0, X1, ToGlobal<K>{node(cur)}, ToLds<K>{X0});
                 }
                 publish();
                 const int nxt = next_toward(cur, tgt);

@@ -796,16 +796,18 @@ struct GroupMerge {
 
     // Outputs of lane threadIdx.x: window base (first output index of its
     // group) and the lane's column j; out[s] is output base + s*G + j.
+    template <bool CP = false>
     __device__ __forceinline__ static void run(const K* a, const K* b, K (&out)[L], int& base, int& j) {
         const int t = threadIdx.x;
         j = t % G;
         base = (t / G) * W;
-        window<L>(a, b, base, j, out);
+        window<L, CP>(a, b, base, j, out);
     }
 
     // One group's window of G*LL outputs starting at output D; lane j gets
     // outputs D + s*G + j (s < LL).  D is the same for the G lanes.
-    template <int LL>
+    // CP: look for copy windows (below) first.
+    template <int LL, bool CP = false>
     __device__ __forceinline__ static void window(const K* a, const K* b, int D, int j, K (&out)[LL]) {
         constexpr int W = G * LL;
         int lo = D > B ? D - B : 0;
@@ -840,6 +842,27 @@ struct GroupMerge {
             hi = nhi;
         }
         const int ai = lo, bi = D - lo;
+        if constexpr (CP) {
+            // A window whose W outputs all come from one input (its next W
+            // keys there are no larger than the other input's next key, or
+            // the other input is used up) is a copy, already sorted: no
+            // compares.  Where a batch of spread keys meets a node near the
+            // root, which holds the smallest keys, that is nearly every
+            // window of both halves.  The four probes are the same for the
+            // whole group (broadcast loads); they cost one LDS round trip,
+            // which the merges of the sift, whose windows are mixed, do not
+            // pay (the delete measured 2-6% slower with them at B <= 4096).
+            const K a_first = a[kmin(ai, B - 1)], a_last = a[kmin(ai + W - 1, B - 1)];
+            const K b_first = b[kmin(bi, B - 1)], b_last = b[kmin(bi + W - 1, B - 1)];
+            const bool pa = (ai + W <= B) & ((bi >= B) | !(b_first < a_last));
+            const bool pb = (bi + W <= B) & ((ai >= B) | !(a_first < b_last));
+            if (pa | pb) {
+                const K* src = (pa ? a + ai : b + bi) + j;
+#pragma unroll
+                for (int s = 0; s < LL; ++s) out[s] = src[s * G];
+                return;
+            }
+        }
         if (ai + W <= B && bi + W <= B) {
             // the window lies inside both inputs (every group but a few near
             // the ends): loads at fixed offsets from two base addresses
@@ -927,12 +950,12 @@ __device__ __forceinline__ void put_strided(const D& dst, int off, int j, const 
 // Merge a and b (LDS) with the group merge and write the B smallest keys to
 // `small`, the B largest to `large`.  Destinations may alias a or b.  Ends
 // with a barrier.
-template <typename K, int B, int NT, typename DS, typename DL, int GS = merge_group<K>()>
+template <typename K, int B, int NT, bool CP = false, typename DS, typename DL, int GS = merge_group<K>()>
 __device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& small, const DL& large) {
     using GM = GroupMerge<K, B, NT, GS>;
     K out[GM::L];
     int base, j;
-    GM::run(a, b, out, base, j);
+    GM::template run<CP>(a, b, out, base, j);
     __syncthreads();
     if (base < B) put_strided<K, GM::L, GM::G>(small, base, j, out);
     else put_strided<K, GM::L, GM::G>(large, base - B, j, out);
@@ -948,7 +971,9 @@ __device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& s
 // B >= 1024 uses the group merge's windows (one workgroup, u32, lower half:
 // 1.14 / 1.30 / 1.85 us at B = 2048 / 4096 / 8192 against 1.55 / 1.82 /
 // 2.51 us for the merge path), smaller B the merge path.
-template <typename K, int B, int NT, bool HI, bool GRP = (B >= 1024)>
+// CP: look for copy windows first (GroupMerge::window), for the merges of
+// an insertion, whose two batches seldom overlap near the root.
+template <typename K, int B, int NT, bool HI, bool CP = false, bool GRP = (B >= 1024)>
 struct HalfMerge {
     static constexpr int N = B / NT;
     static constexpr bool kGroup = GRP;
@@ -959,7 +984,7 @@ struct HalfMerge {
     __device__ __forceinline__ void run(const K* a, const K* b) {
         const int t = threadIdx.x;
         if constexpr (kGroup) {
-            GroupMerge<K, B, NT>::template window<N>(a, b, (HI ? B : 0) + (t / G) * (G * N), t % G, v);
+            GroupMerge<K, B, NT>::template window<N, CP>(a, b, (HI ? B : 0) + (t / G) * (G * N), t % G, v);
         } else {
             merge_path_at<K, B, N>(a, b, (HI ? B : 0) + t * N, v);
         }
@@ -983,19 +1008,20 @@ struct HalfMerge {
 //   B = 8192: the two group-window halves one after the other (3.55 us
 //     against 4.52 for the one-pass group merge, whose 16 keys per lane
 //     crowd the registers).
-template <typename K, int B, int NT, typename DS, typename DL>
+// CP: copy windows (GroupMerge::window), for B >= 2048.
+template <typename K, int B, int NT, bool CP = false, typename DS, typename DL>
 __device__ __forceinline__ void wg_merge(K* a, K* b, const DS& small, const DL& large) {
     if constexpr (B >= 8192) {
-        HalfMerge<K, B, NT, false> lo;
+        HalfMerge<K, B, NT, false, CP> lo;
         lo.run(a, b);
-        HalfMerge<K, B, NT, true> hi;
+        HalfMerge<K, B, NT, true, CP> hi;
         hi.run(a, b);
         __syncthreads();  // destinations may alias a or b
         lo.put(small);
         hi.put(large);
         __syncthreads();
     } else if constexpr (B >= 2048) {
-        wg_merge_grp<K, B, NT>(a, b, small, large);
+        wg_merge_grp<K, B, NT, CP>(a, b, small, large);
     } else {
         wg_merge_bitonic<K, B, NT>(a, b, small, large);
     }

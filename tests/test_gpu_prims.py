@@ -48,7 +48,8 @@ def test_sort_batch(gpu, B, key):
 
 @pytest.mark.parametrize("B", BATCHES)
 @pytest.mark.parametrize("key", ["u32", "u64"])
-@pytest.mark.parametrize("algo", [0, 1, 2, 3, 4, 5], ids=["bitonic", "merge_path", "group", "serial", "halves", "lean"])
+@pytest.mark.parametrize("algo", [0, 1, 2, 3, 4, 5, 6, 7],
+                         ids=["bitonic", "merge_path", "group", "serial", "halves", "lean", "halves_copy", "group_copy"])
 def test_merge_batches(gpu, B, key, algo):
     kt, dtype = KEYS[key]
     if key == "u64" and B == 8192:
@@ -65,6 +66,15 @@ def test_merge_batches(gpu, B, key, algo):
     top = np.iinfo(dtype).max  # keys equal to the padding value of the kernels
     a[4][-(B // 4):] = top
     b[4][-(B // 3):] = top
+    # a node near the root against a batch of spread keys, a few of them
+    # below the node's largest (mostly copy windows), both ways round, and
+    # one key of the spread batch equal to the node's largest
+    a[5] = np.arange(B, dtype=dtype) * dtype(3)
+    few = np.array([1, 3 * B // 2, 3 * (B - 1)], dtype=dtype)
+    b[5] = np.sort(np.concatenate([few, rng.integers(3 * B, int(top), B - 3, dtype=np.uint64).astype(dtype)]))
+    a[6], b[6] = b[5].copy(), a[5].copy()
+    a[7] = a[5]
+    b[7] = np.sort(np.concatenate([a[5][-1:], rng.integers(3 * B, int(top), B - 1, dtype=np.uint64).astype(dtype)]))
     d = [DeviceBuffer.from_numpy(x) for x in (a, b)]
     small, large = DeviceBuffer(a.nbytes), DeviceBuffer(a.nbytes)
     check(lib().bgpq_test_merge(kt, B, d[0].ptr, d[1].ptr, small.ptr, large.ptr, nb, algo))
