@@ -681,6 +681,11 @@ struct DevHeap {
     // measured 2% faster than one bitonic merge per step and the plain walk
     // once the copy windows came in (4% slower without them).
     static constexpr bool kSplitWalk = BU && B >= 1024;
+    // The walk's merge groups: the half merges' default, except that the
+    // partial-buffer insertion at B = 4096, whose kernel already holds 128
+    // registers, keeps 32-lane groups (64-lane ones made it spill and its
+    // insert 12% slower).
+    static constexpr int kWalkGroup = (PB && B == 4096) ? merge_group<K>() : 0;
 
     // bu: lanes 0 and 1 request the locks of the two children of `nxt`
     // without waiting; the state word found (INUSE if another workgroup holds
@@ -773,12 +778,12 @@ struct DevHeap {
     // the smaller half (fold_small, from the same LDS inputs).
     static constexpr bool kSplitFold = B >= kSplitFrom;
     __device__ __forceinline__ void fold_large(const K* X0, const K* X1) const {
-        HalfMerge<K, B, NT, true> hi;
+        HalfMerge<K, B, NT, true, false, merge_group<K>()> hi;
         hi.run(X0, X1);
         hi.put(ToGlobal<K>{node(0)});
     }
     __device__ __forceinline__ void fold_small(K* X0, const K* X1) const {
-        HalfMerge<K, B, NT, false> lo;
+        HalfMerge<K, B, NT, false, false, merge_group<K>()> lo;
         lo.run(X0, X1);
         __syncthreads();  // every read of X0 is done
         lo.put(ToLds<K>{X0});
@@ -1289,13 +1294,13 @@ struct DevHeap {
                         publish();
                         if (threadIdx.x == 0) release_st(par, par_word);
                     } else {
-                        HalfMerge<K, B, NT, false, true> lo;
+                        HalfMerge<K, B, NT, false, true, kWalkGroup> lo;
                         lo.run(C, P);
                         lo.put(ToGlobal<K>{node(par)});
                         lo.put(ToLds<K>{S});
                         publish();
                         if (threadIdx.x == 0) release_st(par, par_word);
-                        HalfMerge<K, B, NT, true, true> hi;
+                        HalfMerge<K, B, NT, true, true, kWalkGroup> hi;
                         hi.run(C, P);
                         hi.put(ToGlobal<K>{node(cur)});
                         K* t = C;  // the parent's new batch is S

@@ -336,18 +336,50 @@ __device__ __forceinline__ void load_run(K (&v)[N], const K* src) {
 // accesses, which the LDS serves 8 lanes at a time; runs 8 words apart put
 // lanes t and t + 4 on the same banks, so lanes of odd quads move their
 // second half first and every access instruction covers all 32 banks.
+// A 32-byte run as two 16-byte words and back, in registers (a run cast
+// to uint4 in a local array, indexed by a per-lane value, went through
+// scratch memory).
+template <typename K, int N>
+__device__ __forceinline__ void run_to_words(const K (&v)[N], uint4& q0, uint4& q1) {
+    static_assert(N * sizeof(K) == 32, "32-byte run");
+    if constexpr (sizeof(K) == 4) {
+        q0 = make_uint4(v[0], v[1], v[2], v[3]);
+        q1 = make_uint4(v[4], v[5], v[6], v[7]);
+    } else {
+        q0 = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
+        q1 = make_uint4((uint32_t)v[2], (uint32_t)(v[2] >> 32), (uint32_t)v[3], (uint32_t)(v[3] >> 32));
+    }
+}
+template <typename K, int N>
+__device__ __forceinline__ void words_to_run(K (&v)[N], const uint4& q0, const uint4& q1) {
+    static_assert(N * sizeof(K) == 32, "32-byte run");
+    if constexpr (sizeof(K) == 4) {
+        v[0] = q0.x; v[1] = q0.y; v[2] = q0.z; v[3] = q0.w;
+        v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
+    } else {
+        v[0] = (K)q0.x | ((K)q0.y << 32);
+        v[1] = (K)q0.z | ((K)q0.w << 32);
+        v[2] = (K)q1.x | ((K)q1.y << 32);
+        v[3] = (K)q1.z | ((K)q1.w << 32);
+    }
+}
+__device__ __forceinline__ uint4 sel_words(bool c, const uint4& x, const uint4& y) {
+    return make_uint4(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z, c ? x.w : y.w);
+}
+
+// The same moves for runs in LDS.  A 32-byte run goes as two 16-byte
+// accesses, which the LDS serves 8 lanes at a time; runs 8 words apart put
+// lanes t and t + 4 on the same banks, so lanes of odd quads move their
+// second half first and every access instruction covers all 32 banks.
 template <typename K, int N>
 __device__ __forceinline__ void store_run_lds(K* dst, const K (&v)[N]) {
     if constexpr (N * sizeof(K) == 32) {
-        K w[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) w[i] = v[i];
-        uint4 q[2];
-        store_run<K, N>(reinterpret_cast<K*>(q), w);
+        uint4 q0, q1;
+        run_to_words<K, N>(v, q0, q1);
         const int sw = ((int)threadIdx.x >> 2) & 1;
         uint4* d = reinterpret_cast<uint4*>(dst);
-        d[sw] = sw ? q[1] : q[0];
-        d[sw ^ 1] = sw ? q[0] : q[1];
+        d[sw] = sel_words(sw, q1, q0);
+        d[sw ^ 1] = sel_words(sw, q0, q1);
     } else {
         store_run<K, N>(dst, v);
     }
@@ -358,10 +390,7 @@ __device__ __forceinline__ void load_run_lds(K (&v)[N], const K* src) {
         const int sw = ((int)threadIdx.x >> 2) & 1;
         const uint4* s = reinterpret_cast<const uint4*>(src);
         const uint4 x = s[sw], y = s[sw ^ 1];
-        uint4 q[2];
-        q[0] = sw ? y : x;
-        q[1] = sw ? x : y;
-        load_run<K, N>(v, reinterpret_cast<const K*>(q));
+        words_to_run<K, N>(v, sel_words(sw, y, x), sel_words(sw, x, y));
     } else {
         load_run<K, N>(v, src);
     }
@@ -833,6 +862,8 @@ struct GroupMerge {
             if constexpr (G == 32) {
                 const int c_lo = __popc((uint32_t)m), c_hi = __popc((uint32_t)(m >> 32));
                 c = shift ? c_hi : c_lo;
+            } else if constexpr (G == 64) {
+                c = __popcll(m);
             } else {
                 c = __popcll((m >> shift) & ((1ull << G) - 1));
             }
@@ -925,7 +956,7 @@ struct GroupMerge {
 };
 
 // Store the group layout (lane j of a G-lane group holds outputs off + s*G + j).
-// A row s of a group is one 128-byte line (G keys of 128/G bytes), so plain
+// A row s of a group is one or two 128-byte lines (G keys), so plain
 // key-sized stores, one per row, already write whole lines; the row's offset
 // is a constant in the instruction.  (A 4x4 transpose inside each quad of
 // lanes into 16-byte runs cost about 32 VALU operations per lane per half
@@ -939,7 +970,7 @@ __device__ __forceinline__ void put_strided(const D& dst, int off, int j, const 
 #pragma unroll
         for (int s = 0; s < L; ++s) dst.p[off + s * G + j] = v[s];
     } else {
-        static_assert(G * sizeof(K) == 128, "a group row is one 128-byte line");
+        static_assert(G * sizeof(K) % 128 == 0, "a group row is whole 128-byte lines");
         const __amdgpu_buffer_rsrc_t rs = buf_rsrc(dst.p);
         const int lane_off = (off + j) * (int)sizeof(K);
 #pragma unroll
@@ -973,18 +1004,24 @@ __device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& s
 // 2.51 us for the merge path), smaller B the merge path.
 // CP: look for copy windows first (GroupMerge::window), for the merges of
 // an insertion, whose two batches seldom overlap near the root.
-template <typename K, int B, int NT, bool HI, bool CP = false, bool GRP = (B >= 1024)>
+// GS: lanes per merge group, 0 for the default: the whole wave for u32 keys
+// up to B = 4096, whose merges are latency-bound (the split search is then
+// uniform and takes a round fewer, for one more cleaner stage through
+// ds_bpermute; heap sort of 2^26 keys: +6% at B = 1024, +5% at 2048 and
+// 4096), 128 bytes of keys otherwise (at B = 8192, 8 keys per lane, the
+// extra stage cost more than the search saves: -4.7%).
+template <typename K, int B, int NT, bool HI, bool CP = false, int GS = 0, bool GRP = (B >= 1024)>
 struct HalfMerge {
     static constexpr int N = B / NT;
     static constexpr bool kGroup = GRP;
-    static constexpr int G = merge_group<K>();
+    static constexpr int G = GS ? GS : (sizeof(K) == 4 && B <= 4096) ? 64 : merge_group<K>();
     static_assert(!kGroup || NT % G == 0, "half merge shape");
     K v[N];
 
     __device__ __forceinline__ void run(const K* a, const K* b) {
         const int t = threadIdx.x;
         if constexpr (kGroup) {
-            GroupMerge<K, B, NT>::template window<N, CP>(a, b, (HI ? B : 0) + (t / G) * (G * N), t % G, v);
+            GroupMerge<K, B, NT, G>::template window<N, CP>(a, b, (HI ? B : 0) + (t / G) * (G * N), t % G, v);
         } else {
             merge_path_at<K, B, N>(a, b, (HI ? B : 0) + t * N, v);
         }
