@@ -846,21 +846,24 @@ struct DevHeap {
                     pre = kNoPre;
                     if (threadIdx.x == 0) bc[BC_E] = 2;
                 } else {
-                    if (threadIdx.x == 0) {
-                        int t2 = 2;
-                        // a child that is an insert target in flight is treated
-                        // as absent (tditdd:425-446)
-                        if (lin) {
-                            const int s = lock_child_td(l);
-                            if (s != AVAIL) t2 = 0;
+                    // a child that is an insert target in flight is treated as
+                    // absent, and so is the right one after an absent left one
+                    // (tditdd:425-446).  Lanes 0 and 1 take the two locks at
+                    // the same time (one round trip instead of two); a right
+                    // lock taken beside an absent left child is given back
+                    // untouched.
+                    int s = AVAIL;
+                    if (threadIdx.x < 2 && (threadIdx.x == 0 ? lin : rin))
+                        s = lock_child_td(threadIdx.x == 0 ? l : r);
+                    if (threadIdx.x < 64) {
+                        const int sr = __shfl(s, 1, 64);
+                        if (threadIdx.x == 0) {
+                            const int t2 = s != AVAIL ? 0 : sr != AVAIL ? 1 : 2;
+                            if (t2 == 0 && rin && sr == AVAIL) a_store(st(r), AVAIL);
+                            bc[BC_C] = AVAIL;
+                            bc[BC_D] = AVAIL;
+                            bc[BC_E] = t2;
                         }
-                        if (t2 == 2 && rin) {
-                            const int s = lock_child_td(r);
-                            if (s != AVAIL) t2 = 1;
-                        }
-                        bc[BC_C] = AVAIL;
-                        bc[BC_D] = AVAIL;
-                        bc[BC_E] = t2;
                     }
                 }
                 __syncthreads();
