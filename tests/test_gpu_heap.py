@@ -175,6 +175,31 @@ def test_heapsort_api(gpu, variant, order):
     assert np.array_equal(d.to_numpy(np.uint32, n), np.sort(keys))
 
 
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("B", [1024, 4096, 8192])
+@pytest.mark.parametrize("order", ["ascending", "descending", "ties"])
+def test_heapsort_orders_large_batches(gpu, variant, B, order):
+    # the batch sizes whose merges use group windows (copy windows in the
+    # insertions, 64-lane groups up to 4096): keys in ascending order (every
+    # insertion stops at once), descending (every window near the root is a
+    # copy) and 12 distinct values (ties at every window edge)
+    print(f"[case] heapsort orders {variant} B={B} {order}", flush=True)
+    n = 1 << 20
+    keys = O.keys_u32(11, n)
+    if order == "ascending":
+        keys = np.sort(keys)
+    elif order == "descending":
+        keys = np.sort(keys)[::-1].copy()
+    else:
+        keys = (keys % np.uint32(12)) * np.uint32(1000)
+    h = Heap(batch_num_for(n, B), B, variant=variant)
+    d = DeviceBuffer.from_numpy(keys)
+    h.heapsort(d, n)
+    assert h.error_flags() == 0 and h.isEmpty()
+    assert np.array_equal(d.to_numpy(np.uint32, n), np.sort(keys))
+    h.close()
+
+
 @pytest.mark.parametrize("variant", PB)
 def test_partial_only_and_small_inputs(gpu, variant):
     # fewer keys than a batch: they live in the partial buffer only, and one
