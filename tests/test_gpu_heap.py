@@ -176,27 +176,28 @@ def test_heapsort_api(gpu, variant, order):
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
-@pytest.mark.parametrize("B", [1024, 4096, 8192])
+@pytest.mark.parametrize("key,B", [("u32", 1024), ("u32", 4096), ("u32", 8192), ("u64", 1024), ("u64", 4096)])
 @pytest.mark.parametrize("order", ["ascending", "descending", "ties"])
-def test_heapsort_orders_large_batches(gpu, variant, B, order):
+def test_heapsort_orders_large_batches(gpu, variant, key, B, order):
     # the batch sizes whose merges use group windows (copy windows in the
-    # insertions, 64-lane groups up to 4096): keys in ascending order (every
-    # insertion stops at once), descending (every window near the root is a
-    # copy) and 12 distinct values (ties at every window edge)
-    print(f"[case] heapsort orders {variant} B={B} {order}", flush=True)
+    # insertions, 64-lane groups for u32 up to 4096): keys in ascending order
+    # (every insertion stops at once), descending (every window near the
+    # root is a copy) and 12 distinct values (ties at every window edge)
+    print(f"[case] heapsort orders {variant} {key} B={B} {order}", flush=True)
     n = 1 << 20
-    keys = O.keys_u32(11, n)
+    keys = O.keys_u64(11, n) if key == "u64" else O.keys_u32(11, n)
+    dt = keys.dtype.type
     if order == "ascending":
         keys = np.sort(keys)
     elif order == "descending":
         keys = np.sort(keys)[::-1].copy()
     else:
-        keys = (keys % np.uint32(12)) * np.uint32(1000)
-    h = Heap(batch_num_for(n, B), B, variant=variant)
+        keys = (keys % dt(12)) * dt(1000)
+    h = Heap(batch_num_for(n, B), B, variant=variant, key=key)
     d = DeviceBuffer.from_numpy(keys)
     h.heapsort(d, n)
     assert h.error_flags() == 0 and h.isEmpty()
-    assert np.array_equal(d.to_numpy(np.uint32, n), np.sort(keys))
+    assert np.array_equal(d.to_numpy(keys.dtype, n), np.sort(keys))
     h.close()
 
 
