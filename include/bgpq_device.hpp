@@ -777,6 +777,9 @@ struct DevHeap {
     // first (fold_large), so status[0] can be released before the batch takes
     // the smaller half (fold_small, from the same LDS inputs).
     static constexpr bool kSplitFold = B >= kSplitFrom;
+    // The fold merges spread keys with spread keys (mixed windows) and keeps
+    // the 128-byte groups: with 64-lane ones `buitdd` measured the same at
+    // B = 1024 and 2048.
     __device__ __forceinline__ void fold_large(const K* X0, const K* X1) const {
         HalfMerge<K, B, NT, true, false, merge_group<K>()> hi;
         hi.run(X0, X1);

@@ -332,10 +332,6 @@ __device__ __forceinline__ void load_run(K (&v)[N], const K* src) {
     }
 }
 
-// The same moves for runs in LDS.  A 32-byte run goes as two 16-byte
-// accesses, which the LDS serves 8 lanes at a time; runs 8 words apart put
-// lanes t and t + 4 on the same banks, so lanes of odd quads move their
-// second half first and every access instruction covers all 32 banks.
 // A 32-byte run as two 16-byte words and back, in registers (a run cast
 // to uint4 in a local array, indexed by a per-lane value, went through
 // scratch memory).
