@@ -672,9 +672,15 @@ struct DevHeap {
     // Smallest batch size whose level steps take the two merge halves one
     // after the other, with the rest of the pipeline below.  Measured heap
     // sort of 2^28 u32 keys (buitdd_np): B = 1024 1.36e8 keys/s split against
-    // 1.26e8 with one bitonic merge per step; B = 512 and 256 are 1% and 4%
-    // slower split, where a half merge costs about as much as a whole one.
-    static constexpr int kSplitFrom = 1024;
+    // 1.26e8 with one bitonic merge per step.  With the half merges on
+    // 64-lane group windows, splitting also pays for u32 keys at B = 512 (the
+    // delete of 2^24 keys 12-16% faster for bu, 4% for td) and B = 256
+    // (7-11%).  u64 keys split from B = 1024 only: at B = 256 the A* and
+    // knapsack solves were 4-6% slower split, and at B = 512 (SSSP 8% faster)
+    // the split sift raises the 256-thread serve kernel from 159 to 179
+    // registers, so it no longer fits on a CU beside the 512-thread one,
+    // which the serve kernels need (each CU holds one workgroup of each).
+    static constexpr int kSplitFrom = sizeof(K) == 4 ? 256 : 1024;
     // bu insertion walk for B >= 1024: two half merges per step, with copy
     // windows, the node's batch stays in LDS between steps (a third buffer)
     // and the parent's load overlaps the node's lock.  At B = 1024 this

@@ -995,7 +995,7 @@ __device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& s
 // can be taken from the same inputs one after the other, and the first can
 // be stored (and its node released) before the second is computed.  Thread
 // 0's v[0] is the half's smallest key and thread NT-1's v[N-1] its largest.
-// B >= 1024 uses the group merge's windows (one workgroup, u32, lower half:
+// B >= 256 uses the group merge's windows (one workgroup, u32, lower half:
 // 1.14 / 1.30 / 1.85 us at B = 2048 / 4096 / 8192 against 1.55 / 1.82 /
 // 2.51 us for the merge path), smaller B the merge path.
 // CP: look for copy windows first (GroupMerge::window), for the merges of
@@ -1006,7 +1006,7 @@ __device__ __forceinline__ void wg_merge_grp(const K* a, const K* b, const DS& s
 // ds_bpermute; heap sort of 2^26 keys: +6% at B = 1024, +5% at 2048 and
 // 4096), 128 bytes of keys otherwise (at B = 8192, 8 keys per lane, the
 // extra stage cost more than the search saves: -4.7%).
-template <typename K, int B, int NT, bool HI, bool CP = false, int GS = 0, bool GRP = (B >= 1024)>
+template <typename K, int B, int NT, bool HI, bool CP = false, int GS = 0, bool GRP = (B >= 256)>
 struct HalfMerge {
     static constexpr int N = B / NT;
     static constexpr bool kGroup = GRP;
